@@ -1,0 +1,524 @@
+// C ABI and host orchestration of one clustering run.
+//
+// The reference drives its path from main() (main.cu:35-459): device reset,
+// ten cudaMallocs, one cudaMalloc per tree node, a host loop that copies the
+// seed lists, collision matrix and all labels back and forth every
+// iteration (main.cu:369-403, dbscan.cu:25-187).  Here a handle owns a
+// device arena that grows on demand and is reused; a run is a fixed sequence
+// of kernel launches on the handle's stream with two small host reads (the
+// sort key width after the grid set-up, and the correction-queue length
+// after each expansion pass).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "dclust.h"
+#include "internal.h"
+#include "kernels.h"
+
+using namespace dclust;
+
+namespace {
+
+constexpr int64_t kChunk = 4096;  // points per chunk of bbox / cell-key blocks
+constexpr int kMaxExpandPasses = 100000;
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+enum Ev { E_START, E_INDEX, E_CORE0, E_CORE, E_EXPAND, E_MERGE, E_BORDER, E_END, E_N };
+
+}  // namespace
+
+struct dclust_handle {
+  int device = 0;
+  int num_cu = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[E_N] = {};
+  std::vector<Buf> bufs;
+  int64_t last_n = -1;
+  uint8_t* last_core = nullptr;  // device, input order
+  int last_err = 0;
+
+  template <class T>
+  T* get(int slot, size_t count, hipError_t* err) {
+    if ((size_t)slot >= bufs.size()) bufs.resize(slot + 1);
+    Buf& b = bufs[slot];
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (b.cap < bytes) {
+      if (b.p) (void)hipFree(b.p);
+      b.p = nullptr;
+      b.cap = 0;
+      size_t want = bytes + bytes / 8;  // headroom for slightly larger runs
+      hipError_t e = hipMalloc(&b.p, want);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        e = hipMalloc(&b.p, bytes);
+        want = bytes;
+      }
+      if (e != hipSuccess) {
+        *err = e;
+        b.p = nullptr;
+        return nullptr;
+      }
+      b.cap = want;
+    }
+    return (T*)b.p;
+  }
+};
+
+namespace {
+
+enum Slot {
+  S_XYZ, S_INST, S_CHUNK, S_BBOX, S_CTL, S_KEY0, S_KEY1, S_VAL0, S_VAL1, S_HIST, S_OFFS,
+  S_DTOT, S_PASS, S_CSTART, S_CKEY, S_CINST, S_CM, S_ROWS, S_CELLOF, S_X0, S_X1, S_X2,
+  S_G0, S_G1, S_G2, S_ORIG, S_CORE, S_LABEL, S_PARENT, S_COMPMIN, S_PAIRS, S_Q0, S_Q1,
+  S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT
+};
+
+int check_params(const dclust_params* p) {
+  if (!p) return DCLUST_E_PARAM;
+  if (p->dim != 2 && p->dim != 3) return DCLUST_E_PARAM;
+  if (!(p->eps > 0.0) || !std::isfinite(p->eps)) return DCLUST_E_PARAM;
+  if (p->minpts < 1) return DCLUST_E_PARAM;
+  if (p->partitions_r < 0 || p->seed_list_size < 0 || p->blocks < 0 || p->block_threads < 0 ||
+      p->extra_collision < 0)
+    return DCLUST_E_PARAM;
+  if (p->block_threads != 0 && p->block_threads != 64 && p->block_threads != 128 &&
+      p->block_threads != 256)
+    return DCLUST_E_PARAM;
+  // seed lists live in LDS: (threads / 64) * seeds * 4 bytes <= 64 KiB
+  if (p->seed_list_size > 4096) return DCLUST_E_PARAM;
+  if (p->precision != DCLUST_PRECISION_FP64 && p->precision != DCLUST_PRECISION_FP32_GUARD)
+    return DCLUST_E_PARAM;
+  return DCLUST_OK;
+}
+
+float ev_ms(dclust_handle* h, int a, int b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, h->ev[a], h->ev[b]) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0.f;
+  }
+  return ms;
+}
+
+struct RunShape {
+  int seeds;
+  int threads;
+  int blocks;
+  int64_t pair_cap;
+};
+
+// Launch shape: explicit parameters win; zeros come from the tuning rule
+// (seed list 1024 entries, 256 threads, blocks at resident occupancy).
+RunShape pick_shape(dclust_handle* h, const dclust_params& p, int64_t n, bool f32) {
+  RunShape r;
+  r.seeds = p.seed_list_size > 0 ? p.seed_list_size : 1024;
+  r.threads = p.block_threads > 0 ? p.block_threads : 256;
+  if (p.blocks > 0) {
+    r.blocks = p.blocks;
+  } else {
+    int per_cu = expand_blocks_per_cu(f32, r.seeds, r.threads);
+    r.blocks = h->num_cu * std::min(per_cu, 8);
+  }
+  r.blocks = std::max(8, (r.blocks + 7) / 8 * 8);  // multiple of 8 (XCD remap)
+  int64_t per = p.extra_collision > 0 ? p.extra_collision : 0;
+  r.pair_cap = per > 0 ? per * r.blocks : std::max<int64_t>(1 << 16, n / 4);
+  return r;
+}
+
+#define HIPCHK(x)                          \
+  do {                                     \
+    hipError_t e_ = (x);                   \
+    if (e_ != hipSuccess) {                \
+      herr = e_;                           \
+      goto fail;                           \
+    }                                      \
+  } while (0)
+
+// One run over `ninst` instances whose coordinates are already on the
+// device (d_xyz, rows back to back) and whose labels go to d_labels.
+int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64_t* n,
+               const dclust_params* params, int32_t* d_labels, int32_t* inst_err,
+               dclust_stats* stats) {
+  hipError_t herr = hipSuccess;
+  int rc = DCLUST_OK;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (ninst <= 0) return DCLUST_E_PARAM;
+  bool f32 = params[0].precision == DCLUST_PRECISION_FP32_GUARD;
+  std::vector<Inst> inst(ninst);
+  std::vector<Chunk> chunks;
+  int64_t ntot = 0, off = 0;
+  for (int32_t i = 0; i < ninst; ++i) {
+    int c = check_params(&params[i]);
+    if (c) return c;
+    if (n[i] < 0) return DCLUST_E_PARAM;
+    if ((params[i].precision == DCLUST_PRECISION_FP32_GUARD) != f32) return DCLUST_E_PARAM;
+    Inst& I = inst[i];
+    std::memset(&I, 0, sizeof(I));
+    I.dim = params[i].dim;
+    I.minpts = params[i].minpts;
+    I.eps = params[i].eps;
+    I.eps2 = params[i].eps * params[i].eps;
+    I.r = params[i].partitions_r;
+    I.pt_begin = ntot;
+    I.pt_end = ntot + n[i];
+    I.xyz_off = off;
+    for (int64_t b = 0; b < n[i]; b += kChunk) {
+      Chunk ch;
+      ch.inst = i;
+      ch.pad = 0;
+      ch.begin = ntot + b;
+      ch.end = ntot + std::min(n[i], b + kChunk);
+      chunks.push_back(ch);
+    }
+    ntot += n[i];
+    off += n[i] * I.dim;
+  }
+  if (ntot >= (int64_t)INT32_MAX - 1) return DCLUST_E_CAPACITY;
+  if (inst_err)
+    for (int32_t i = 0; i < ninst; ++i) inst_err[i] = 0;
+  if (stats) stats->n_points = ntot;
+  if (ntot == 0) return DCLUST_OK;
+
+  const RunShape shape = pick_shape(h, params[0], ntot, f32);
+  const int nchunks = (int)chunks.size();
+  const int grid_cells = h->num_cu * 8;
+  const hipStream_t s = h->stream;
+
+  Dev d;
+  std::memset(&d, 0, sizeof(d));
+  d.n = ntot;
+  d.ninst = ninst;
+  d.nchunks = nchunks;
+  d.xyz = d_xyz;
+  d.pair_cap = shape.pair_cap;
+  {
+    hipError_t& E = herr;
+    d.inst = h->get<Inst>(S_INST, ninst, &E);
+    d.chunks = h->get<Chunk>(S_CHUNK, nchunks, &E);
+    d.bbox = h->get<unsigned long long>(S_BBOX, (size_t)ninst * 8, &E);
+    d.ctl = h->get<long long>(S_CTL, C_NCTL, &E);
+    d.key[0] = h->get<uint64_t>(S_KEY0, ntot, &E);
+    d.key[1] = h->get<uint64_t>(S_KEY1, ntot, &E);
+    d.val[0] = h->get<int32_t>(S_VAL0, ntot, &E);
+    d.val[1] = h->get<int32_t>(S_VAL1, ntot, &E);
+    d.hist = h->get<uint32_t>(S_HIST, sort_hist_entries(ntot), &E);
+    d.offs = h->get<uint32_t>(S_OFFS, sort_hist_entries(ntot), &E);
+    d.dtot = h->get<uint32_t>(S_DTOT, 256, &E);
+    d.pass_scatter = h->get<int32_t>(S_PASS, kMaxPasses, &E);
+    d.cell_start = h->get<int32_t>(S_CSTART, ntot + 1, &E);
+    d.cell_key = h->get<uint64_t>(S_CKEY, ntot, &E);
+    d.cell_inst = h->get<int32_t>(S_CINST, ntot, &E);
+    d.cell_m = h->get<int32_t>(S_CM, ntot, &E);
+    d.rows = h->get<int2>(S_ROWS, (size_t)ntot * kMaxRows, &E);
+    d.cell_of = h->get<int32_t>(S_CELLOF, ntot, &E);
+    d.X[0] = h->get<double>(S_X0, ntot, &E);
+    d.X[1] = h->get<double>(S_X1, ntot, &E);
+    d.X[2] = h->get<double>(S_X2, ntot, &E);
+    d.G[0] = h->get<uint32_t>(S_G0, ntot, &E);
+    d.G[1] = h->get<uint32_t>(S_G1, ntot, &E);
+    d.G[2] = h->get<uint32_t>(S_G2, ntot, &E);
+    d.orig = h->get<int32_t>(S_ORIG, ntot, &E);
+    d.core = h->get<uint8_t>(S_CORE, ntot, &E);
+    d.label = h->get<int32_t>(S_LABEL, ntot, &E);
+    d.parent = h->get<int32_t>(S_PARENT, ntot, &E);
+    d.compmin = h->get<int32_t>(S_COMPMIN, ntot, &E);
+    d.pairs = h->get<int2>(S_PAIRS, shape.pair_cap, &E);
+    d.q[0] = h->get<int32_t>(S_Q0, ntot, &E);
+    d.q[1] = h->get<int32_t>(S_Q1, ntot, &E);
+    d.keyv = h->get<int32_t>(S_KEYV, ntot, &E);
+    d.flag = h->get<uint8_t>(S_FLAG, ntot, &E);
+    d.rank = h->get<int32_t>(S_RANK, ntot + 1, &E);
+    d.scan_tmp = h->get<int64_t>(S_SCAN, scan_tmp_entries(ntot), &E);
+    d.core_out = h->get<uint8_t>(S_COREOUT, ntot, &E);
+    if (herr != hipSuccess) return DCLUST_E_OOM;
+  }
+  d.labels_out = d_labels;
+
+  {
+    long long ctl_h[C_NCTL];
+    int passes = 0;
+    int npass = 0;
+    HIPCHK(hipMemcpyAsync(d.inst, inst.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync((void*)d.chunks, chunks.data(), sizeof(Chunk) * nchunks,
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(h->ev[E_START], s));
+    launch_fill_i64(d.ctl, C_NCTL, 0, s);
+    launch_fill_i32(d.pass_scatter, kMaxPasses, 0, s);
+    launch_bbox_init(d, s);
+    launch_index_head(d, nchunks, s);
+    HIPCHK(hipMemcpyAsync(ctl_h, d.ctl, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (ctl_h[C_ERR]) {
+      HIPCHK(hipMemcpy(inst.data(), d.inst, sizeof(Inst) * ninst, hipMemcpyDeviceToHost));
+      int first = 0;
+      for (int32_t i = 0; i < ninst; ++i) {
+        if (inst_err) inst_err[i] = inst[i].err;
+        if (inst[i].err && !first) first = inst[i].err;
+      }
+      if (ninst == 1 || !inst_err) return first ? first : DCLUST_E_PARAM;
+      rc = first;  // batched with per-instance errors: keep going
+    }
+    passes = (int)((ctl_h[C_KEYBITS] + 7) / 8);
+    if (stats) stats->key_bits = (int32_t)ctl_h[C_KEYBITS];
+    launch_index_tail(d, ntot, nchunks, grid_cells, passes, s);
+    HIPCHK(hipEventRecord(h->ev[E_INDEX], s));
+
+    launch_fill_i32(d.label, ntot, kUnclaimed, s);
+    launch_iota(d.parent, ntot, s);
+    launch_fill_i32(d.compmin, ntot, kNoKey, s);
+    launch_fill_u8(d.flag, ntot, 0, s);
+    // the core (neighbour-count) kernel is bracketed by its own events
+    HIPCHK(hipEventRecord(h->ev[E_CORE0], s));
+    launch_core(d, f32, grid_cells, s);
+    HIPCHK(hipEventRecord(h->ev[E_CORE], s));
+
+    for (npass = 0; npass < kMaxExpandPasses; ++npass) {
+      launch_pass_prep(d, npass, s);
+      launch_expand(d, f32, npass, shape.blocks, shape.seeds, shape.threads, s);
+      long long qn = 0;
+      HIPCHK(hipMemcpyAsync(&qn, d.ctl + C_Q0_N + ((npass + 1) & 1), sizeof(qn),
+                            hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (qn == 0) break;
+    }
+    ++npass;
+    HIPCHK(hipEventRecord(h->ev[E_EXPAND], s));
+    launch_merge(d, grid_cells, s);
+    HIPCHK(hipEventRecord(h->ev[E_MERGE], s));
+    launch_border(d, f32, grid_cells, ntot, s);
+    HIPCHK(hipEventRecord(h->ev[E_BORDER], s));
+    launch_relabel(d, ntot, s);
+    HIPCHK(hipEventRecord(h->ev[E_END], s));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctl_h, d.ctl, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    h->last_n = ntot;
+    h->last_core = d.core_out;
+    if (stats) {
+      stats->t_index_ms = ev_ms(h, E_START, E_CORE0);
+      stats->t_core_ms = ev_ms(h, E_CORE0, E_CORE);
+      stats->t_expand_ms = ev_ms(h, E_CORE, E_EXPAND);
+      stats->t_merge_ms = ev_ms(h, E_EXPAND, E_MERGE);
+      stats->t_border_ms = ev_ms(h, E_MERGE, E_BORDER);
+      stats->t_relabel_ms = ev_ms(h, E_BORDER, E_END);
+      stats->t_total_ms = ev_ms(h, E_START, E_END);
+      stats->n_cells = ctl_h[C_NCELL];
+      stats->n_core = ctl_h[C_NCORE];
+      stats->n_noise = ctl_h[C_NNOISE];
+      stats->n_clusters = ctl_h[C_NCLUST];
+      stats->n_chains = ctl_h[C_NCHAIN];
+      stats->n_collisions = ctl_h[C_NPAIR];
+      stats->n_overflow = ctl_h[C_OVF];
+      stats->n_passes = npass;
+      stats->cand_total = ctl_h[C_CAND];
+      stats->nbr_cells_total = ctl_h[C_NBRC];
+      stats->band_rechecks = ctl_h[C_BAND];
+      stats->band_flips = ctl_h[C_FLIP];
+      stats->asym_pairs = ctl_h[C_ASYM];
+      stats->pairs_evaluated = ctl_h[C_EVAL];
+    }
+    if (npass >= kMaxExpandPasses) return DCLUST_E_CAPACITY;
+    return rc;
+  }
+fail:
+  (void)hipGetLastError();
+  if (stats) stats->hip_error = (int32_t)herr;
+  h->last_err = (int)herr;
+  return DCLUST_E_HIP;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dclust_params_default(dclust_params* p) {
+  if (!p) return DCLUST_E_PARAM;
+  std::memset(p, 0, sizeof(*p));
+  p->dim = 2;        // DIMENSION
+  p->eps = 1.5;      // EPS
+  p->minpts = 3;     // MINPTS
+  p->partitions_r = 3;  // PARTITION_SIZE
+  p->seed_list_size = 0;
+  p->blocks = 0;
+  p->block_threads = 0;
+  p->extra_collision = 0;
+  p->precision = DCLUST_PRECISION_FP64;
+  return DCLUST_OK;
+}
+
+int dclust_create(dclust_handle** out, int device) {
+  if (!out) return DCLUST_E_PARAM;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return DCLUST_E_NODEVICE;
+  }
+  if (device < 0 || device >= ndev) return DCLUST_E_PARAM;
+  dclust_handle* h = new (std::nothrow) dclust_handle();
+  if (!h) return DCLUST_E_OOM;
+  h->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete h;
+    return DCLUST_E_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete h;
+    return DCLUST_E_HIP;
+  }
+  h->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return DCLUST_E_HIP;
+  }
+  for (auto& e : h->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete h;
+      return DCLUST_E_HIP;
+    }
+  *out = h;
+  return DCLUST_OK;
+}
+
+void dclust_destroy(dclust_handle* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto& b : h->bufs)
+    if (b.p) (void)hipFree(b.p);
+  for (auto& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int dclust_run_device(dclust_handle* h, const double* d_xyz, int64_t n, const dclust_params* p,
+                      int32_t* d_labels_out, dclust_stats* stats) {
+  if (!h || !p || n < 0 || (n > 0 && (!d_xyz || !d_labels_out))) return DCLUST_E_PARAM;
+  if (hipSetDevice(h->device) != hipSuccess) return DCLUST_E_HIP;
+  return run_device(h, 1, d_xyz, &n, p, d_labels_out, nullptr, stats);
+}
+
+int dclust_run(dclust_handle* h, const double* xyz, int64_t n, const dclust_params* p,
+               int32_t* labels_out, dclust_stats* stats) {
+  if (!h || !p || n < 0 || (n > 0 && (!xyz || !labels_out))) return DCLUST_E_PARAM;
+  int c = check_params(p);
+  if (c) return c;
+  if (n == 0) {
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    return DCLUST_OK;
+  }
+  if (hipSetDevice(h->device) != hipSuccess) return DCLUST_E_HIP;
+  hipError_t herr = hipSuccess;
+  double* dx = h->get<double>(S_XYZ, (size_t)n * p->dim, &herr);
+  int32_t* dl = h->get<int32_t>(S_LABOUT, n, &herr);
+  if (!dx || !dl) return DCLUST_E_OOM;
+  if (hipMemcpyAsync(dx, xyz, sizeof(double) * n * p->dim, hipMemcpyHostToDevice, h->stream) !=
+      hipSuccess)
+    return DCLUST_E_HIP;
+  int rc = run_device(h, 1, dx, &n, p, dl, nullptr, stats);
+  if (rc != DCLUST_OK) return rc;
+  if (hipMemcpyAsync(labels_out, dl, sizeof(int32_t) * n, hipMemcpyDeviceToHost, h->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    return DCLUST_E_HIP;
+  return DCLUST_OK;
+}
+
+int dclust_run_batched_device(dclust_handle* h, int32_t count, const double* d_xyz,
+                              const int64_t* n, const dclust_params* params, int32_t* d_labels,
+                              int32_t* inst_err, dclust_stats* stats) {
+  if (!h || count <= 0 || !n || !params || !d_xyz || !d_labels) return DCLUST_E_PARAM;
+  if (hipSetDevice(h->device) != hipSuccess) return DCLUST_E_HIP;
+  return run_device(h, count, d_xyz, n, params, d_labels, inst_err, stats);
+}
+
+int dclust_run_batched(dclust_handle* h, int32_t count, const double* const* xyz, const int64_t* n,
+                       const dclust_params* params, int32_t* const* labels, int32_t* inst_err,
+                       dclust_stats* stats) {
+  if (!h || count <= 0 || !xyz || !n || !params || !labels) return DCLUST_E_PARAM;
+  if (hipSetDevice(h->device) != hipSuccess) return DCLUST_E_HIP;
+  int64_t tot = 0, el = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    int c = check_params(&params[i]);
+    if (c) return c;
+    if (n[i] < 0 || (n[i] > 0 && (!xyz[i] || !labels[i]))) return DCLUST_E_PARAM;
+    tot += n[i];
+    el += n[i] * params[i].dim;
+  }
+  if (tot == 0) {
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (inst_err)
+      for (int32_t i = 0; i < count; ++i) inst_err[i] = 0;
+    return DCLUST_OK;
+  }
+  hipError_t herr = hipSuccess;
+  double* dx = h->get<double>(S_XYZ, el, &herr);
+  int32_t* dl = h->get<int32_t>(S_LABOUT, tot, &herr);
+  if (!dx || !dl) return DCLUST_E_OOM;
+  int64_t e = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    if (n[i] > 0 && hipMemcpyAsync(dx + e, xyz[i], sizeof(double) * n[i] * params[i].dim,
+                                   hipMemcpyHostToDevice, h->stream) != hipSuccess)
+      return DCLUST_E_HIP;
+    e += n[i] * params[i].dim;
+  }
+  int rc = run_device(h, count, dx, n, params, dl, inst_err, stats);
+  if (rc != DCLUST_OK && !(inst_err && rc < 0 && rc != DCLUST_E_HIP && rc != DCLUST_E_OOM))
+    return rc;
+  int64_t o = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    if (n[i] > 0 && hipMemcpyAsync(labels[i], dl + o, sizeof(int32_t) * n[i],
+                                   hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+      return DCLUST_E_HIP;
+    o += n[i];
+  }
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return DCLUST_E_HIP;
+  return rc;
+}
+
+int dclust_last_core_flags(dclust_handle* h, uint8_t* core_out, int64_t n) {
+  if (!h || !core_out || n != h->last_n || !h->last_core) return DCLUST_E_PARAM;
+  if (hipMemcpy(core_out, h->last_core, (size_t)n, hipMemcpyDeviceToHost) != hipSuccess)
+    return DCLUST_E_HIP;
+  return DCLUST_OK;
+}
+
+int dclust_autotune(dclust_handle* h, int64_t n, dclust_params* p) {
+  if (!h || !p) return DCLUST_E_PARAM;
+  bool f32 = p->precision == DCLUST_PRECISION_FP32_GUARD;
+  RunShape r = pick_shape(h, *p, n, f32);
+  if (p->seed_list_size == 0) p->seed_list_size = r.seeds;
+  if (p->block_threads == 0) p->block_threads = r.threads;
+  if (p->blocks == 0) p->blocks = r.blocks;
+  if (p->extra_collision == 0) p->extra_collision = (int32_t)std::min<int64_t>(INT32_MAX, r.pair_cap / r.blocks);
+  return DCLUST_OK;
+}
+
+const char* dclust_strerror(int code) {
+  switch (code) {
+    case DCLUST_OK: return "ok";
+    case DCLUST_E_PARAM: return "invalid parameter or non-finite input";
+    case DCLUST_E_BINWIDTH: return "cell width (min extent / r) is less than eps";
+    case DCLUST_E_IO: return "dataset or label file error";
+    case DCLUST_E_HIP: return "HIP runtime error";
+    case DCLUST_E_OOM: return "out of memory";
+    case DCLUST_E_CAPACITY: return "size limit exceeded";
+    case DCLUST_E_NODEVICE: return "no usable GPU";
+    default: return "unknown error";
+  }
+}
+
+const char* dclust_version(void) { return "dclust-mi355x 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,669 @@
+// Grid-index build on gfx950: the MI355X replacement for the reference's
+// tree index (indexing.cu:23-118, main.cu:166-340).
+//
+//   k_bbox        per-instance bounding box (main.cu:166-181 does it on the
+//                 host, initialised to 0 / 999999999; here +-inf)
+//   k_grid_setup  cell width, cells per axis, key space per instance
+//                 (main.cu:184-207 bin width + eps check, main.cu:209-244 tree
+//                 metadata: replaced by a flat grid)
+//   k_cellkey     per-point cell key (insertData, indexing.cu:107-118)
+//   radix sort    LDS radix sort of (key, id), stable (thrust::sort_by_key,
+//                 main.cu:332-333)
+//   CSR           wavefront ballot/mbcnt compaction of key changes into cell
+//                 ranges (INDEXING_ADJUSTMENT's per-leaf equal_range,
+//                 indexing.cu:23-51)
+//   k_gather      cell-sorted SoA coordinates (FP64 + fixed-point)
+//   k_rows        the 3^(D-1) neighbour rows of each cell as point ranges
+//                 (searchPoints' level-by-level descent, dbscan.cu:390-474)
+#include "internal.h"
+#include "kernels.h"
+
+#include <cmath>
+
+namespace dclust {
+
+// ------------------------------------------------------------------ fill --
+template <class T>
+__global__ void k_fill(T* p, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_iota(int32_t* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (int32_t)i;
+}
+
+static int fill_grid(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill<int32_t>, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v);
+}
+void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill<uint8_t>, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v);
+}
+void launch_fill_i64(long long* p, int64_t n, long long v, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill<long long>, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v);
+}
+void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v);
+}
+void launch_iota(int32_t* p, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_iota, dim3(fill_grid(n)), dim3(256), 0, s, p, n);
+}
+
+// ------------------------------------------------------------------ bbox --
+// slots per instance: 0..2 min (ordered keys, start at the largest key),
+// 3..5 max, 6 max |x|, 7 non-finite flag (all start at 0)
+__global__ void k_bbox_init(Dev d) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)d.ninst * 8;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d.bbox[i] = (i & 7) < 3 ? ~0ull : 0ull;
+}
+
+void launch_bbox_init(const Dev& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_bbox_init, dim3(fill_grid((int64_t)d.ninst * 8)), dim3(256), 0, s, d);
+}
+
+// One block per chunk (chunks never straddle instances).  Block reduction,
+// then one ordered-u64 atomic min/max per axis.  Non-finite input sets the
+// instance error.
+__global__ __launch_bounds__(256) void k_bbox(Dev d) {
+  const Chunk ch = d.chunks[blockIdx.x];
+  const Inst& I = d.inst[ch.inst];
+  const int dim = I.dim;
+  double mn[3] = {INFINITY, INFINITY, INFINITY};
+  double mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  double ma = 0.0;
+  bool bad = false;
+  for (int64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
+    const double* p = d.xyz + I.xyz_off + (i - I.pt_begin) * dim;
+    for (int k = 0; k < dim; ++k) {
+      double v = p[k];
+      bad |= !isfinite(v);
+      mn[k] = fmin(mn[k], v);
+      mx[k] = fmax(mx[k], v);
+      ma = fmax(ma, fabs(v));
+    }
+  }
+  __shared__ double s_red[4][7];
+  __shared__ int s_bad;
+  if (threadIdx.x == 0) s_bad = 0;
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = wave_min(mn[k]);
+    mx[k] = wave_max(mx[k]);
+  }
+  ma = wave_max(ma);
+  bool wb = ballot(bad) != 0;
+  __syncthreads();
+  const int wv = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    for (int k = 0; k < 3; ++k) {
+      s_red[wv][k] = mn[k];
+      s_red[wv][3 + k] = mx[k];
+    }
+    s_red[wv][6] = ma;
+    if (wb) atomicOr(&s_bad, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+      for (int k = 0; k < 3; ++k) {
+        s_red[0][k] = fmin(s_red[0][k], s_red[w][k]);
+        s_red[0][3 + k] = fmax(s_red[0][3 + k], s_red[w][3 + k]);
+        s_red[0][6] = fmax(s_red[0][6], s_red[w][6]);
+      }
+    unsigned long long* bb = d.bbox + (int64_t)ch.inst * 8;
+    for (int k = 0; k < dim; ++k) {
+      atomicMin(&bb[k], ord_d(s_red[0][k]));
+      atomicMax(&bb[3 + k], ord_d(s_red[0][3 + k]));
+    }
+    atomicMax(&bb[6], ord_d(s_red[0][6]));
+    if (s_bad) atomicOr((unsigned long long*)&bb[7], 1ULL);
+  }
+}
+
+// ------------------------------------------------------------ grid setup --
+// Cells are cubes of width w: w = max(min extent / r, eps*(1+2^-20)) when r
+// is given (DCLUST_E_BINWIDTH if min extent / r < eps, as main.cu:204-207
+// refuses), w = eps*(1+2^-20) otherwise.  The 2^-20 margin keeps every
+// eps-box inside the 3^D cells around its centre despite rounding of the
+// cell function (the rounding error of floor(fl(fl(x-mn)*inv_w)) is below
+// 5*2^-53 * cells-per-axis).  Axes are capped at 2^21 cells (3-D) or 2^31
+// (2-D) by widening w, which only coarsens the grid.
+//
+// FP32 form: distances are formed from fixed-point cell offsets in units of
+// w*2^-30.  The FP32 squared distance D2 differs from the exact squared
+// distance (same units) by at most 5*2^-24 relative (int->float conversion,
+// square, two fused adds) plus the quantisation of the offsets.  Pairs with
+// |D2 - eps^2| <= band are re-decided with the exact FP64 predicate; band is
+// 8 ulp = 2^-20 relative unless the bound above (plus the box-test slack
+// from coordinate magnitude) requires more, in which case it is widened to
+// twice the bound.
+__global__ void k_grid_setup(Dev d) {
+  __shared__ long long s_scan[1024];
+  __shared__ long long s_carry;
+  __shared__ int s_err;
+  if (threadIdx.x == 0) {
+    s_carry = 0;
+    s_err = 0;
+  }
+  __syncthreads();
+  for (int base = 0; base < d.ninst; base += blockDim.x) {
+    const int i = base + threadIdx.x;
+    long long cells = 0;
+    if (i < d.ninst) {
+      Inst I = d.inst[i];
+      const unsigned long long* bb = d.bbox + (int64_t)i * 8;
+      const int dim = I.dim;
+      double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+      double emin = INFINITY;
+      bool empty = I.pt_end <= I.pt_begin;
+      for (int k = 0; k < dim; ++k) {
+        mn[k] = empty ? 0.0 : unord_d(bb[k]);
+        mx[k] = empty ? 0.0 : unord_d(bb[3 + k]);
+        emin = fmin(emin, mx[k] - mn[k]);
+      }
+      I.maxabs = empty ? 0.0 : unord_d(bb[6]);
+      if (bb[7]) I.err = -1;  // DCLUST_E_PARAM: non-finite input
+      const double wmin = I.eps * (1.0 + 0x1p-20);
+      double w = wmin;
+      if (I.r > 0) {
+        double wr = emin / (double)I.r;
+        if (!(wr >= I.eps) && I.err == 0 && !empty) I.err = -2;  // DCLUST_E_BINWIDTH
+        if (wr > w) w = wr;
+      }
+      const double cap = dim == 3 ? 2097152.0 : 2147483647.0;
+      for (;;) {
+        double inv = 1.0 / w;
+        bool ok = true;
+        for (int k = 0; k < dim; ++k)
+          if (floor((mx[k] - mn[k]) * inv) + 1.0 > cap) ok = false;
+        if (ok) break;
+        w *= 2.0;
+      }
+      I.w = w;
+      I.inv_w = 1.0 / w;
+      cells = 1;
+      for (int k = 0; k < 3; ++k) {
+        I.mn[k] = k < dim ? mn[k] : 0.0;
+        I.nk[k] = 1;
+        if (k < dim) {
+          double t = floor(cell_t(mx[k], mn[k], I.inv_w));
+          I.nk[k] = (int32_t)t + 1;
+          cells *= (long long)I.nk[k];
+        }
+      }
+      // FP32 guard band (units of w * 2^-30)
+      const double gs = I.inv_w * 0x1p30;
+      const double es = I.eps * gs;
+      const double e2 = es * es;
+      const double u = 0x1p-24;
+      double bound = 5.0 * u * (1.0 + 4.0 * u);
+      bound += 2.0 * 1.7320508075688772 * 2.5 / es + 6.25 * 3.0 / e2;  // quantisation
+      bound += 8.0 * 0x1p-52 * (I.maxabs / I.eps + 1.0);             // box slack
+      double band = 0x1p-20;
+      if (bound > 0.5 * band) band = 2.0 * bound;
+      float fin = (float)(e2 * (1.0 - band));
+      if ((double)fin > e2 * (1.0 - band)) fin = __int_as_float(__float_as_int(fin) - 1);
+      float fout = (float)(e2 * (1.0 + band));
+      if ((double)fout < e2 * (1.0 + band)) fout = __int_as_float(__float_as_int(fout) + 1);
+      I.d2_in = fin;
+      I.d2_out = fout;
+      I.d2_mid = (float)e2;
+      d.inst[i] = I;
+      if (I.err) atomicOr(&s_err, 1);
+    }
+    s_scan[threadIdx.x] = cells;
+    __syncthreads();
+    // inclusive Hillis-Steele scan over the block
+    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+      long long v = threadIdx.x >= (unsigned)o ? s_scan[threadIdx.x - o] : 0;
+      __syncthreads();
+      s_scan[threadIdx.x] += v;
+      __syncthreads();
+    }
+    if (i < d.ninst) d.inst[i].key_base = s_carry + s_scan[threadIdx.x] - cells;
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) s_carry += s_scan[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    long long total = s_carry;
+    int bits = 1;
+    while (bits < 63 && (1LL << bits) < total) ++bits;
+    d.ctl[C_KEYBITS] = bits;
+    d.ctl[C_ERR] = s_err;
+  }
+}
+
+// --------------------------------------------------------------- cellkey --
+__device__ __forceinline__ int32_t cell_index(double x, const Inst& I, int k) {
+  double t = floor(cell_t(x, I.mn[k], I.inv_w));
+  int32_t c = (t >= 0.0) ? (int32_t)fmin(t, (double)(I.nk[k] - 1)) : 0;
+  return c;
+}
+
+__global__ __launch_bounds__(256) void k_cellkey(Dev d) {
+  const Chunk ch = d.chunks[blockIdx.x];
+  const Inst& I = d.inst[ch.inst];
+  const int dim = I.dim;
+  for (int64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
+    const double* p = d.xyz + I.xyz_off + (i - I.pt_begin) * dim;
+    int64_t lin = cell_index(p[0], I, 0);
+    for (int k = 1; k < dim; ++k) lin = lin * I.nk[k] + cell_index(p[k], I, k);
+    d.key[0][i] = (uint64_t)(I.key_base + lin);
+    d.val[0][i] = (int32_t)i;
+  }
+}
+
+// ------------------------------------------------------------ radix sort --
+// LSD radix sort, 8-bit digits, 4096-key tiles (256 threads x 16 keys).
+// Buffer parity: the number of scatters executed so far.
+constexpr int RS_T = 256;
+constexpr int RS_I = 16;
+constexpr int RS_TILE = RS_T * RS_I;
+
+__device__ __forceinline__ int sort_parity(const Dev& d, int pass) {
+  int p = 0;
+  for (int q = 0; q < pass; ++q) p += d.pass_scatter[q];
+  return p & 1;
+}
+
+// peers of this lane with the same 8-bit digit among `valid` lanes
+__device__ __forceinline__ unsigned long long match8(uint32_t dg, bool valid) {
+  unsigned long long peers = ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    bool bit = (dg >> b) & 1;
+    unsigned long long m = ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
+
+__device__ __forceinline__ bool pass_active(const Dev& d, int pass) {
+  return pass * 8 < (int)d.ctl[C_KEYBITS];
+}
+
+__global__ __launch_bounds__(RS_T) void k_rs_hist(Dev d, int pass, int nblk) {
+  if (!pass_active(d, pass)) return;
+  const uint64_t* keys = d.key[sort_parity(d, pass)];
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int shift = pass * 8;
+  const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_I; ++r) {
+    int64_t i = t0 + r * RS_T + threadIdx.x;
+    bool v = i < d.n;
+    uint32_t dg = v ? (uint32_t)(keys[i] >> shift) & 255u : 0u;
+    unsigned long long peers = match8(dg, v);
+    if (v && mbcnt(peers) == 0) atomicAdd(&h[dg], (uint32_t)__popcll(peers));
+  }
+  __syncthreads();
+  d.hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Digit totals (one block per digit), then digit bases + pass decision,
+// then per-digit row scans.  offs[d][b] = global start of digit d in block b.
+__global__ __launch_bounds__(256) void k_rs_total(Dev d, int pass, int nblk) {
+  if (!pass_active(d, pass)) return;
+  const uint32_t* row = d.hist + (int64_t)blockIdx.x * nblk;
+  uint32_t s = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += row[b];
+  s = wave_sum(s);
+  __shared__ uint32_t red[4];
+  if (lane_id() == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) d.dtot[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void k_rs_base(Dev d, int pass) {
+  __shared__ uint32_t s[256];
+  __shared__ int s_skip;
+  if (!pass_active(d, pass)) {
+    if (threadIdx.x == 0) d.pass_scatter[pass] = 0;
+    return;
+  }
+  uint32_t v = d.dtot[threadIdx.x];
+  if (threadIdx.x == 0) s_skip = 0;
+  __syncthreads();
+  if ((int64_t)v == d.n) s_skip = 1;  // every key has this digit
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    uint32_t u = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
+    __syncthreads();
+    s[threadIdx.x] += u;
+    __syncthreads();
+  }
+  d.dtot[threadIdx.x] = s[threadIdx.x] - v;  // exclusive digit base
+  if (threadIdx.x == 0) d.pass_scatter[pass] = s_skip ? 0 : 1;
+}
+
+__global__ __launch_bounds__(256) void k_rs_rowscan(Dev d, int pass, int nblk) {
+  if (!pass_active(d, pass) || !d.pass_scatter[pass]) return;
+  const uint32_t* row = d.hist + (int64_t)blockIdx.x * nblk;
+  uint32_t* out = d.offs + (int64_t)blockIdx.x * nblk;
+  __shared__ uint32_t s_w[4];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = d.dtot[blockIdx.x];
+  __syncthreads();
+  for (int b0 = 0; b0 < nblk; b0 += blockDim.x) {
+    int b = b0 + threadIdx.x;
+    uint32_t v = b < nblk ? row[b] : 0;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane_id() == 63) s_w[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    uint32_t pre = s_carry;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) pre += s_w[w];
+    if (b < nblk) out[b] = pre + inc - v;
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry += s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    __syncthreads();
+  }
+}
+
+// Stable scatter: ranks are computed round by round (256 consecutive keys
+// per round, waves in order, lanes in order) so equal digits keep their
+// input order; the tile is staged in LDS in local digit order and written
+// out coalesced per digit run.
+__global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) {
+  if (!pass_active(d, pass) || !d.pass_scatter[pass]) return;
+  const int par = sort_parity(d, pass);
+  const uint64_t* kin = d.key[par];
+  const int32_t* vin = d.val[par];
+  uint64_t* kout = d.key[par ^ 1];
+  int32_t* vout = d.val[par ^ 1];
+  const int shift = pass * 8;
+  __shared__ uint64_t s_k[RS_TILE];
+  __shared__ int32_t s_v[RS_TILE];
+  __shared__ uint32_t s_lstart[256], s_run[256], s_goff[256];
+  __shared__ uint32_t s_wc[4][256];
+  const int tid = threadIdx.x, wv = tid >> 6;
+  const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
+  {
+    uint32_t c = d.hist[(int64_t)tid * nblk + blockIdx.x];
+    s_goff[tid] = d.offs[(int64_t)tid * nblk + blockIdx.x];
+    s_run[tid] = 0;
+    for (int w = 0; w < 4; ++w) s_wc[w][tid] = 0;
+    s_lstart[tid] = c;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+      uint32_t u = tid >= o ? s_lstart[tid - o] : 0;
+      __syncthreads();
+      s_lstart[tid] += u;
+      __syncthreads();
+    }
+    s_lstart[tid] -= c;
+    __syncthreads();
+  }
+  for (int r = 0; r < RS_I; ++r) {
+    int64_t i = t0 + r * RS_T + tid;
+    bool v = i < d.n;
+    uint64_t key = v ? kin[i] : 0;
+    int32_t val = v ? vin[i] : 0;
+    uint32_t dg = (uint32_t)(key >> shift) & 255u;
+    unsigned long long peers = match8(dg, v);
+    int rank = mbcnt(peers);
+    int cnt = __popcll(peers);
+    if (v && rank == 0) s_wc[wv][dg] = cnt;
+    __syncthreads();
+    if (v) {
+      uint32_t pre = s_run[dg];
+      for (int w = 0; w < wv; ++w) pre += s_wc[w][dg];
+      uint32_t lp = s_lstart[dg] + pre + rank;
+      s_k[lp] = key;
+      s_v[lp] = val;
+    }
+    __syncthreads();
+    if (v && rank == 0) {
+      atomicAdd(&s_run[dg], (uint32_t)cnt);
+      s_wc[wv][dg] = 0;
+    }
+    __syncthreads();
+  }
+  const int64_t tn = (d.n - t0) < RS_TILE ? (d.n - t0) : RS_TILE;
+  for (int l = tid; l < tn; l += RS_T) {
+    uint64_t key = s_k[l];
+    uint32_t dg = (uint32_t)(key >> shift) & 255u;
+    int64_t g = (int64_t)s_goff[dg] + (l - s_lstart[dg]);
+    kout[g] = key;
+    vout[g] = s_v[l];
+  }
+}
+
+// ------------------------------------------------------- CSR compaction --
+// Flag = sorted key differs from its predecessor.  Pass 1 counts flags per
+// 4096-key tile with ballot/popcount; a single block scans the tile counts;
+// pass 2 ranks flags inside the tile with mbcnt and writes the cell ranges.
+constexpr int CS_T = 256;
+constexpr int CS_I = 16;
+constexpr int CS_TILE = CS_T * CS_I;
+
+__device__ __forceinline__ const uint64_t* sorted_keys(const Dev& d) {
+  return d.key[sort_parity(d, kMaxPasses)];
+}
+
+__global__ __launch_bounds__(CS_T) void k_csr_count(Dev d) {
+  const uint64_t* key = sorted_keys(d);
+  const int64_t t0 = (int64_t)blockIdx.x * CS_TILE;
+  int c = 0;
+  for (int r = 0; r < CS_I; ++r) {
+    int64_t i = t0 + r * CS_T + threadIdx.x;
+    bool f = i < d.n && (i == 0 || key[i] != key[i - 1]);
+    unsigned long long m = ballot(f);
+    c += __popcll(m);
+  }
+  __shared__ int s[4];
+  if (lane_id() == 0) s[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) d.scan_tmp[blockIdx.x] = s[0] + s[1] + s[2] + s[3];
+}
+
+// exclusive scan of nb int64 values in place by one block; writes the total
+// to *total (may be null)
+__global__ __launch_bounds__(1024) void k_scan_top(int64_t* v, int nb, long long* total) {
+  __shared__ int64_t s[1024];
+  const int per = (nb + 1023) / 1024;
+  const int b0 = threadIdx.x * per;
+  int64_t sum = 0;
+  for (int j = 0; j < per; ++j)
+    if (b0 + j < nb) sum += v[b0 + j];
+  s[threadIdx.x] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int64_t u = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
+    __syncthreads();
+    s[threadIdx.x] += u;
+    __syncthreads();
+  }
+  int64_t run = s[threadIdx.x] - sum;
+  for (int j = 0; j < per; ++j)
+    if (b0 + j < nb) {
+      int64_t x = v[b0 + j];
+      v[b0 + j] = run;
+      run += x;
+    }
+  if (threadIdx.x == 1023 && total) *total = s[1023];
+}
+
+// instance owning a key: last i with key_base <= key
+__device__ __forceinline__ int32_t inst_of_key(const Dev& d, uint64_t key) {
+  int lo = 0, hi = d.ninst - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if ((uint64_t)d.inst[mid].key_base <= key) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(CS_T) void k_csr_write(Dev d) {
+  const uint64_t* key = sorted_keys(d);
+  const int64_t t0 = (int64_t)blockIdx.x * CS_TILE;
+  __shared__ int s_w[4];
+  __shared__ int64_t s_run;
+  const int wv = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_run = d.scan_tmp[blockIdx.x];
+  __syncthreads();
+  for (int r = 0; r < CS_I; ++r) {
+    int64_t i = t0 + r * CS_T + threadIdx.x;
+    bool inr = i < d.n;
+    uint64_t k = inr ? key[i] : 0;
+    bool f = inr && (i == 0 || k != key[i - 1]);
+    unsigned long long m = ballot(f);
+    if (lane_id() == 0) s_w[wv] = __popcll(m);
+    __syncthreads();
+    int64_t pre = s_run;
+    for (int w = 0; w < wv; ++w) pre += s_w[w];
+    int64_t cid = pre + mbcnt(m) + (f ? 1 : 0) - 1;  // inclusive - 1
+    if (inr) {
+      d.cell_of[i] = (int32_t)cid;
+      if (f) {
+        d.cell_start[cid] = (int32_t)i;
+        d.cell_key[cid] = k;
+        d.cell_inst[cid] = inst_of_key(d, k);
+      }
+      if (i == d.n - 1) {
+        d.cell_start[cid + 1] = (int32_t)d.n;
+        d.ctl[C_NCELL] = cid + 1;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_run += s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- gather --
+__global__ __launch_bounds__(256) void k_gather(Dev d) {
+  const int32_t* vsrc = d.val[sort_parity(d, kMaxPasses)];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < d.n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t src = vsrc[i];
+    const Inst& I = d.inst[d.cell_inst[d.cell_of[i]]];
+    const int dim = I.dim;
+    const double* p = d.xyz + I.xyz_off + ((int64_t)src - I.pt_begin) * dim;
+    for (int k = 0; k < dim; ++k) {
+      double x = p[k];
+      d.X[k][i] = x;
+      double t = cell_t(x, I.mn[k], I.inv_w);       // same rounding as the key
+      double f = floor(t * 0x1p30);                  // exact scaling by 2^30
+      d.G[k][i] = (uint32_t)(unsigned long long)(long long)f;  // wraps mod 4 cells
+    }
+    d.orig[i] = src;
+  }
+}
+
+// ------------------------------------------------------------------ rows --
+__device__ __forceinline__ int32_t lower_bound_key(const uint64_t* a, int32_t n, uint64_t v) {
+  int32_t lo = 0, hi = n;
+  while (lo < hi) {
+    int32_t mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// For each cell: the 3^(D-1) rows of neighbour cells (all axes but the last
+// fixed at offsets -1/0/+1, the last axis spanning k-1..k+1), as contiguous
+// point ranges.  Row 0 is the cell's own row.
+__global__ __launch_bounds__(256) void k_rows(Dev d) {
+  const int32_t C = (int32_t)d.ctl[C_NCELL];
+  long long cand = 0, nbrc = 0;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < C;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const Inst& I = d.inst[d.cell_inst[c]];
+    const int dim = I.dim;
+    int64_t lin = (int64_t)(d.cell_key[c] - (uint64_t)I.key_base);
+    int32_t kc[3];
+    for (int k = dim - 1; k >= 0; --k) {
+      kc[k] = (int32_t)(lin % I.nk[k]);
+      lin /= I.nk[k];
+    }
+    const int last = dim - 1;
+    const int nrow = dim == 3 ? 9 : 3;
+    const int32_t nc = d.cell_start[c + 1] - d.cell_start[c];
+    int32_t m = 0, kcnt = 0;
+    for (int r = 0; r < nrow; ++r) {
+      // row order: centre first, then the others
+      int o0, o1 = 0;
+      if (dim == 3) {
+        static constexpr int8_t off[9][2] = {{0, 0}, {-1, -1}, {-1, 0}, {-1, 1}, {0, -1},
+                                             {0, 1},  {1, -1},  {1, 0},  {1, 1}};
+        o0 = off[r][0];
+        o1 = off[r][1];
+      } else {
+        static constexpr int8_t off2[3] = {0, -1, 1};
+        o0 = off2[r];
+      }
+      int32_t rc[3] = {kc[0] + o0, dim == 3 ? kc[1] + o1 : kc[1], dim == 3 ? kc[2] : 0};
+      bool ok = rc[0] >= 0 && rc[0] < I.nk[0];
+      if (dim == 3) ok = ok && rc[1] >= 0 && rc[1] < I.nk[1];
+      int2 range = make_int2(0, 0);
+      if (ok) {
+        int32_t lo_l = kc[last] > 0 ? kc[last] - 1 : 0;
+        int32_t hi_l = kc[last] + 1 < I.nk[last] ? kc[last] + 1 : I.nk[last] - 1;
+        int64_t base = rc[0];
+        if (dim == 3) base = base * I.nk[1] + rc[1];
+        base *= I.nk[last];
+        uint64_t klo = (uint64_t)(I.key_base + base + lo_l);
+        uint64_t khi = (uint64_t)(I.key_base + base + hi_l);
+        int32_t a = lower_bound_key(d.cell_key, C, klo);
+        int32_t b = a;
+        while (b < C && d.cell_key[b] <= khi) ++b;
+        range = make_int2(d.cell_start[a], d.cell_start[b]);
+        kcnt += b - a;
+      }
+      d.rows[c * kMaxRows + r] = range;
+      m += range.y - range.x;
+    }
+    d.cell_m[c] = m;
+    cand += (long long)nc * m;
+    nbrc += kcnt;
+  }
+  wave_add(&d.ctl[C_CAND], cand);
+  wave_add(&d.ctl[C_NBRC], nbrc);
+}
+
+// ------------------------------------------------------------- launchers --
+void launch_index_head(const Dev& d, int nchunks, hipStream_t s) {
+  hipLaunchKernelGGL(k_bbox, dim3(nchunks), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(1024), 0, s, d);
+}
+
+void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
+                       int passes, hipStream_t s) {
+  hipLaunchKernelGGL(k_cellkey, dim3(nchunks), dim3(256), 0, s, d);
+  const int nblk = (int)((n + RS_TILE - 1) / RS_TILE);
+  for (int p = 0; p < passes; ++p) {
+    hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
+    hipLaunchKernelGGL(k_rs_total, dim3(256), dim3(256), 0, s, d, p, nblk);
+    hipLaunchKernelGGL(k_rs_base, dim3(1), dim3(256), 0, s, d, p);
+    hipLaunchKernelGGL(k_rs_rowscan, dim3(256), dim3(256), 0, s, d, p, nblk);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
+  }
+  const int ncs = (int)((n + CS_TILE - 1) / CS_TILE);
+  hipLaunchKernelGGL(k_csr_count, dim3(ncs), dim3(CS_T), 0, s, d);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, d.scan_tmp, ncs, (long long*)nullptr);
+  hipLaunchKernelGGL(k_csr_write, dim3(ncs), dim3(CS_T), 0, s, d);
+  hipLaunchKernelGGL(k_gather, dim3(fill_grid(n)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_rows, dim3(grid_cells), dim3(256), 0, s, d);
+}
+
+void launch_scan_top(int64_t* v, int nb, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, v, nb, (long long*)nullptr);
+}
+
+int64_t sort_hist_entries(int64_t n) { return 256 * ((n + RS_TILE - 1) / RS_TILE); }
+int64_t scan_tmp_entries(int64_t n) { return (n + CS_TILE - 1) / CS_TILE + 16; }
+
+}  // namespace dclust

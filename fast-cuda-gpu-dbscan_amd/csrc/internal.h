@@ -1,0 +1,265 @@
+// Internal device/host definitions shared by the HIP translation units.
+//
+// Data layout in HBM (all arrays live in the handle's arena, sized for the
+// largest run seen so far and reused):
+//   input   xyz       caller's row-major FP64 rows, instances back to back
+//   sort    key[2]    u64 cell keys (ping-pong), val[2] i32 point ids
+//   CSR     cell_start[C+1], cell_key[C], cell_inst[C], rows[C][9] (int2
+//           point ranges of the 3^(D-1) neighbour rows, centre row first)
+//   points  (cell-sorted)  X[3][N] FP64 SoA, G[3][N] u32 fixed-point
+//           coordinates (30 fraction bits per cell, wrapping mod 4 cells),
+//           orig[N], cell_of[N], core[N] u8, label[N] (chain id)
+//   merge   parent[N] chain union-find, compmin[N], pairs[P] int2
+//   output  keyv[N] (component min input index), flag[N] u8, rank[N]
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dclust {
+
+constexpr int kWave = 64;
+constexpr int32_t kUnclaimed = -1;
+constexpr int32_t kNoKey = 0x7fffffff;
+constexpr int kFixBits = 30;  // fixed-point fraction bits per cell width
+constexpr int kMaxRows = 9;   // 3^(D-1) neighbour rows for D <= 3
+
+// Per-instance parameters and derived grid (device table, one per instance).
+struct Inst {
+  double mn[3];    // grid origin = bounding-box minimum
+  double inv_w;    // 1 / cell width: cell(x) = floor(fl(fl(x - mn) * inv_w))
+  double w;        // cell width
+  double eps;      // EPS
+  double eps2;     // fl(eps * eps), the oracle's threshold
+  double maxabs;   // max |coordinate| (FP error bounds)
+  float d2_in;     // FP32 form: D2 <  d2_in  -> neighbour
+  float d2_out;    // FP32 form: D2 >  d2_out -> not a neighbour
+  float d2_mid;    // FP32 form: rounded eps^2 in fixed-point units
+  int32_t dim;
+  int32_t minpts;
+  int32_t r;       // PARTITION_SIZE (0 = auto)
+  int32_t nk[3];   // cells per axis
+  int32_t err;     // per-instance error code (DCLUST_E_*)
+  int64_t key_base;            // first key of this instance's cell space
+  int64_t pt_begin, pt_end;    // global point-index range
+  int64_t xyz_off;             // element offset of the first coordinate
+};
+
+// Chunk of points of one instance (host-built work list for per-point
+// kernels that need the instance, so no point->instance search is needed).
+struct Chunk {
+  int32_t inst;
+  int32_t pad;
+  int64_t begin, end;
+};
+
+// Control block (int64 counters in device memory).
+enum Ctl : int {
+  C_KEYBITS = 0,
+  C_NCELL,
+  C_NCHAIN,
+  C_NPAIR,
+  C_Q0_N,
+  C_Q1_N,
+  C_Q0_HEAD,
+  C_Q1_HEAD,
+  C_NCORE,
+  C_NNOISE,
+  C_NCLUST,
+  C_CAND,
+  C_NBRC,
+  C_BAND,
+  C_FLIP,
+  C_ASYM,
+  C_EVAL,
+  C_OVF,
+  C_ERR,
+  C_SORTPAR,  // number of executed radix scatters (buffer parity)
+  C_NCTL = 32
+};
+
+// Sort-pass bookkeeping: pass p is active when p*8 < key bits; its scatter
+// runs unless every key has the same digit (then the data is already in
+// order and the buffers are not swapped).
+constexpr int kMaxPasses = 8;
+
+// All device pointers of one run, passed to kernels by value.
+struct Dev {
+  int64_t n;
+  int32_t ninst;
+  int32_t nchunks;
+  const double* xyz;
+  Inst* inst;
+  const Chunk* chunks;
+  unsigned long long* bbox;  // [ninst][6] ordered-u64 min0..2 max0..2
+  long long* ctl;
+
+  // radix sort
+  uint64_t* key[2];
+  int32_t* val[2];
+  uint32_t* hist;   // [256][nblk] digit-major block histograms
+  uint32_t* offs;   // [256][nblk] scanned offsets
+  uint32_t* dtot;   // [256]
+  int32_t* pass_scatter;  // [kMaxPasses] 1 if the scatter of pass p ran
+
+  // CSR
+  int32_t* cell_start;  // [N+1]
+  uint64_t* cell_key;   // [N]
+  int32_t* cell_inst;   // [N]
+  int32_t* cell_m;      // [N] candidate points in the neighbourhood
+  int2* rows;           // [N][kMaxRows]
+  int32_t* cell_of;     // [N] per sorted point
+
+  // sorted point data
+  double* X[3];
+  uint32_t* G[3];
+  int32_t* orig;
+  uint8_t* core;
+  int32_t* label;
+  int32_t* parent;
+  int32_t* compmin;
+  int2* pairs;
+  int64_t pair_cap;
+  int32_t* q[2];
+  int32_t* keyv;
+  uint8_t* flag;
+  int32_t* rank;
+  int64_t* scan_tmp;  // block sums for scans
+
+  // outputs (input order)
+  int32_t* labels_out;
+  uint8_t* core_out;
+};
+
+// ------------------------------------------------------------ helpers ----
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned long long ballot(bool p) {
+  return __ballot(p);
+}
+
+__device__ __forceinline__ int mbcnt(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi(
+      (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ int rl_i(int v, int l) {
+  return __builtin_amdgcn_readlane(v, l);
+}
+__device__ __forceinline__ uint32_t rl_u(uint32_t v, int l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ float rl_f(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double rl_d(double v, int l) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ int rfl_i(int v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// wave barrier + wavefront-scope fence: orders LDS traffic between lanes of
+// one wave (seed-list pushes followed by pops).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_min(T v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    T u = __shfl_xor(v, o, 64);
+    v = u < v ? u : v;
+  }
+  return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    T u = __shfl_xor(v, o, 64);
+    v = u > v ? u : v;
+  }
+  return v;
+}
+// inclusive prefix sum over the wave
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const int l = lane_id();
+  for (int o = 1; o < 64; o <<= 1) {
+    T u = __shfl_up(v, o, 64);
+    if (l >= o) v += u;
+  }
+  return v;
+}
+
+// Aggregated atomic add of a wave-uniform sum (one atomic per wave).
+__device__ __forceinline__ void wave_add(long long* p, long long v) {
+  long long s = wave_sum(v);
+  if (lane_id() == 0 && s != 0) atomicAdd((unsigned long long*)p, (unsigned long long)s);
+}
+
+// Monotone map double -> u64 (for atomic min/max of doubles).
+__device__ __forceinline__ unsigned long long ord_d(double x) {
+  unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ULL);
+}
+__host__ __device__ inline double unord_d(unsigned long long u) {
+  unsigned long long b = (u >> 63) ? (u & 0x7fffffffffffffffULL) : ~u;
+  double d;
+  __builtin_memcpy(&d, &b, sizeof(d));
+  return d;
+}
+
+// Blocks of a launch are dealt to the 8 XCDs round-robin (block b runs on
+// XCD b % 8).  Remap so that each XCD owns a contiguous range of logical
+// blocks: neighbouring work then shares that XCD's L2.  G must be a
+// multiple of 8.
+__device__ __forceinline__ int xcd_block(int b, int G) {
+  int per = G >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
+// Cell index along one axis, shared by key computation, gather and every
+// search (the reference computes insert and search bins with different
+// formulas, indexing.cu:114 vs indexing.cu:96-101; here there is one).
+__device__ __forceinline__ double cell_t(double x, double mn, double inv_w) {
+  return (x - mn) * inv_w;
+}
+
+// ---- chain union-find (lock-free; roots are the minimum id) ------------
+__device__ __forceinline__ int32_t uf_find(int32_t* par, int32_t x) {
+  for (;;) {
+    int32_t p = __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (p == x) return x;
+    int32_t gp = __hip_atomic_load(&par[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (gp != p)
+      __hip_atomic_store(&par[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = p;
+  }
+}
+
+__device__ __forceinline__ void uf_union(int32_t* par, int32_t a, int32_t b) {
+  for (;;) {
+    a = uf_find(par, a);
+    b = uf_find(par, b);
+    if (a == b) return;
+    if (a < b) {
+      int32_t t = a;
+      a = b;
+      b = t;
+    }
+    if (atomicCAS(&par[a], a, b) == a) return;
+  }
+}
+
+}  // namespace dclust

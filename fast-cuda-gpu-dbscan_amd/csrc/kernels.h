@@ -1,0 +1,36 @@
+// Host-side launchers of the HIP kernels (one declaration per phase).
+#pragma once
+#include "internal.h"
+
+namespace dclust {
+
+void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
+void launch_fill_i64(long long* p, int64_t n, long long v, hipStream_t s);
+void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
+void launch_iota(int32_t* p, int64_t n, hipStream_t s);
+
+void launch_bbox_init(const Dev& d, hipStream_t s);
+// bbox + grid set-up (per-instance cell width, cells per axis, key space)
+void launch_index_head(const Dev& d, int nchunks, hipStream_t s);
+// cell keys, `passes` radix passes, CSR compaction, gather, neighbour rows
+void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
+                       int passes, hipStream_t s);
+// exclusive in-place scan of nb int64 values by one block
+void launch_scan_top(int64_t* v, int nb, hipStream_t s);
+
+int64_t sort_hist_entries(int64_t n);
+int64_t scan_tmp_entries(int64_t n);
+
+void launch_core(const Dev& d, bool f32, int grid, hipStream_t s);
+void launch_pass_prep(const Dev& d, int pass, hipStream_t s);
+void launch_expand(const Dev& d, bool f32, int pass, int grid, int seeds,
+                   int threads, hipStream_t s);
+void launch_merge(const Dev& d, int grid, hipStream_t s);
+void launch_border(const Dev& d, bool f32, int grid, int64_t n, hipStream_t s);
+void launch_relabel(const Dev& d, int64_t n, hipStream_t s);
+
+// occupancy helper for the expansion kernel
+int expand_blocks_per_cu(bool f32, int seeds, int threads);
+
+}  // namespace dclust

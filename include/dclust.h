@@ -1,0 +1,179 @@
+/*
+ * dclust -- MI355X-native CUDA-DClust+ DBSCAN, C ABI (libdclust.so).
+ *
+ * The reference has no library API: its CUDA-DClust+ path is one program,
+ * `main.exe <dataset-file>` (main.cu:35-46), configured by compile-time
+ * macros (common.h:6-25), reading a CSV dataset (main.cu:469-498) and
+ * running index build (indexing.cu:23-118), the seed-list expansion kernel
+ * (dbscan.cu:254-387) and the host merge loop (dbscan.cu:25-187,
+ * main.cu:353-403).  This header is the boundary that replaces that program
+ * for callers: plain C types, no HIP types, error codes instead of exit().
+ *
+ * Labels: labels[i] = 0 for noise, 1..K for clusters, with cluster ids
+ * numbered by the smallest input index of a core point in each cluster.
+ * This is exactly the numbering of the FP64 sequential reference DBSCAN
+ * (cpu-dbscan/dbscan_rtree_exp.cpp:360-406, noise -1 written as 0 as
+ * test-results/cluster-check.cpp:120-139 reads it).  A border point takes
+ * the smallest id among the clusters of the core points it is within eps of.
+ *
+ * Threading: a handle is used by one host thread at a time; distinct
+ * handles may run concurrently.  Every call returns after its results are
+ * complete (labels on the host for the host entry points, on the device for
+ * the *_device entry points).
+ */
+#ifndef DCLUST_H
+#define DCLUST_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (the reference exits instead: common.h:31-40,        ---- */
+/* ---- main.cu:204-207, main.cu:469-474)                                ---- */
+#define DCLUST_OK 0
+#define DCLUST_E_PARAM -1    /* invalid parameter or non-finite input       */
+#define DCLUST_E_BINWIDTH -2 /* cell width (extent / r) < eps, main.cu:204  */
+#define DCLUST_E_IO -3       /* dataset / label file error, main.cu:469     */
+#define DCLUST_E_HIP -4      /* HIP runtime error (see dclust_stats.hip_error) */
+#define DCLUST_E_OOM -5      /* device or host allocation failed            */
+#define DCLUST_E_CAPACITY -6 /* a size limit of this build was exceeded     */
+#define DCLUST_E_NODEVICE -7 /* no usable GPU                               */
+
+/* precision forms */
+#define DCLUST_PRECISION_FP64 0       /* plain FP64 predicate, bit-exact     */
+#define DCLUST_PRECISION_FP32_GUARD 1 /* FP32 distances, FP64 recheck in band */
+
+/*
+ * Parameters.  Names follow common.h:6-25 one to one; 0 selects the
+ * library's tuned value where noted.
+ *   dim             DIMENSION (2 or 3)
+ *   eps             EPS (> 0)
+ *   minpts          MINPTS (>= 1; |N(p)| counts p itself, dbscan.cu:353-369)
+ *   partitions_r    PARTITION_SIZE: cells per axis of the shortest extent,
+ *                   cell width = min extent / r; must give width >= eps
+ *                   (main.cu:192-207).  0 = cell width eps (auto).
+ *   seed_list_size  MAX_SEEDS: seed-list entries per chain (0 = auto).
+ *   blocks          THREAD_BLOCKS: workgroups of the expansion kernel
+ *                   (0 = fill the device at resident occupancy).
+ *   block_threads   THREAD_COUNT: threads per workgroup, multiple of 64
+ *                   (0 = 256).
+ *   extra_collision EXTRA_COLLISION_SIZE: collision-pair buffer entries per
+ *                   workgroup (0 = auto).  Pairs beyond it are merged inline.
+ *   precision       DCLUST_PRECISION_*.
+ */
+typedef struct dclust_params {
+  int32_t dim;
+  int32_t minpts;
+  double eps;
+  int32_t partitions_r;
+  int32_t seed_list_size;
+  int32_t blocks;
+  int32_t block_threads;
+  int32_t extra_collision;
+  int32_t precision;
+} dclust_params;
+
+/* Per-run diagnostics.  Times are HIP-event milliseconds on the handle's
+ * stream (the reference's clock() buckets, main.cu:410-432). */
+typedef struct dclust_stats {
+  double t_total_ms;   /* first kernel to labels resident on the device   */
+  double t_index_ms;   /* bbox + cell keys + radix sort + CSR + rows        */
+  double t_core_ms;    /* neighbour-count (core flag) kernel                */
+  double t_expand_ms;  /* seed-list expansion passes                        */
+  double t_merge_ms;   /* union-find merge of collision pairs               */
+  double t_border_ms;  /* border assignment                                 */
+  double t_relabel_ms; /* canonical relabel + output                        */
+  int64_t n_points;
+  int64_t n_cells;      /* non-empty grid cells                             */
+  int64_t n_core;
+  int64_t n_noise;
+  int64_t n_clusters;
+  int64_t n_chains;     /* chains started by the expansion kernel           */
+  int64_t n_collisions; /* collision pairs recorded                         */
+  int64_t n_overflow;   /* points pushed past a full seed list (corrected)  */
+  int64_t n_passes;     /* expansion passes                                 */
+  int64_t cand_total;   /* sum over cells of n_c * (points in 3^D nbhd)     */
+  int64_t nbr_cells_total; /* sum over cells of non-empty neighbour cells   */
+  int64_t band_rechecks;   /* FP32 form: pairs re-decided in FP64           */
+  int64_t band_flips;      /* FP32 form: rechecks where FP32 alone was wrong */
+  int64_t asym_pairs;      /* core-core pairs with P(p,q) != P(q,p)         */
+  int64_t pairs_evaluated; /* distance evaluations in the core kernel       */
+  int32_t hip_error;       /* last hipError_t, 0 if none                    */
+  int32_t key_bits;        /* bits of the sort key                          */
+} dclust_stats;
+
+typedef struct dclust_handle dclust_handle;
+
+/* common.h defaults: dim 2, eps 1.5, minpts 3, r 3, 128 seeds, 8 blocks of
+ * 16 threads, 256 extra collisions -- except that blocks/threads/seeds/
+ * extra_collision are set to 0 (auto) because the reference's 8x16 launch
+ * is a debugging size. */
+int dclust_params_default(dclust_params* p);
+
+/* Opens a handle on HIP device `device`.  Replaces the device set-up and
+ * allocations of main.cu:72-152 (done lazily and reused across calls). */
+int dclust_create(dclust_handle** h, int device);
+void dclust_destroy(dclust_handle* h);
+
+/* Cluster n points of dimension p->dim given as host row-major doubles
+ * (the ImportDataset layout, main.cu:469-498).  labels_out: n int32 on the
+ * host.  Replaces main.cu:155-432 (index, expand/merge loop, result).
+ * stats may be NULL. */
+int dclust_run(dclust_handle* h, const double* xyz, int64_t n,
+               const dclust_params* p, int32_t* labels_out,
+               dclust_stats* stats);
+
+/* Same with device-resident input and output (HIP device pointers). */
+int dclust_run_device(dclust_handle* h, const double* d_xyz, int64_t n,
+                      const dclust_params* p, int32_t* d_labels_out,
+                      dclust_stats* stats);
+
+/* Cluster `count` independent instances in one pass over the device
+ * (no counterpart in the reference, which clusters one dataset per process).
+ * xyz[i]: n[i] rows of params[i].dim doubles on the host; labels[i]: n[i]
+ * int32 on the host, numbered per instance.  inst_err (may be NULL)
+ * receives a per-instance error code.  stats aggregates the batch. */
+int dclust_run_batched(dclust_handle* h, int32_t count,
+                       const double* const* xyz, const int64_t* n,
+                       const dclust_params* params, int32_t* const* labels,
+                       int32_t* inst_err, dclust_stats* stats);
+
+/* Batched with device-resident data: d_xyz holds the instances' rows back
+ * to back (instance i starts at element sum_{j<i} n[j]*dim[j]); d_labels
+ * receives sum n[i] labels in the same order.  n/params are host arrays. */
+int dclust_run_batched_device(dclust_handle* h, int32_t count,
+                              const double* d_xyz, const int64_t* n,
+                              const dclust_params* params, int32_t* d_labels,
+                              int32_t* inst_err, dclust_stats* stats);
+
+/* Core flags (1 = core) of the last run, in input order, copied to the
+ * host; n must equal the last run's total point count. */
+int dclust_last_core_flags(dclust_handle* h, uint8_t* core_out, int64_t n);
+
+/* Autotuned launch shape for a problem (seed list, blocks, threads); fills
+ * the zero fields of *p from the library's tuning table. */
+int dclust_autotune(dclust_handle* h, int64_t n, dclust_params* p);
+
+/* Reads a CSV dataset with the grammar of ImportDataset (main.cu:469-498):
+ * lines of comma-separated numbers, all values flattened row-major, dim
+ * values per point.  Unlike the reference (fixed DATASET_COUNT, garbage on
+ * short files), the point count comes from the file and a field that does
+ * not parse is an error.  With out == NULL only *n_points is computed.
+ * max_points <= 0 means no limit. */
+int dclust_import_dataset(const char* path, int32_t dim, int64_t max_points,
+                          double* out, int64_t* n_points);
+
+/* Writes one label per line (0 = noise), the file format read by
+ * test-results/cluster-check.cpp:201-224. */
+int dclust_write_labels(const char* path, const int32_t* labels, int64_t n);
+
+const char* dclust_strerror(int code);
+
+/* Library version string. */
+const char* dclust_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,114 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bar: labels byte-identical to the oracle's canonical labels (cluster id =
+rank of the smallest core input index, border = smallest adjacent id,
+noise = 0), core flags bit-identical, in both precision forms.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+import dclust
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+PRECS = [dclust.FP64, dclust.FP32_GUARD]
+EX = np.array([[1, 1], [2, 2], [3, 3], [10, 10], [11, 11], [12, 12], [20, 20], [21, 21],
+               [22, 22]], dtype=np.float64)
+
+
+def _check(handle, x, eps, minpts, prec, **kw):
+    lab = handle.run(x, eps, minpts, precision=prec, **kw)
+    ref, core, asym = oracle.fast_dbscan(x, eps, minpts)
+    assert asym == 0
+    got_core = handle.core_flags(x.shape[0])
+    nbad_core = int(np.sum(got_core != core))
+    nbad = int(np.sum(lab != ref))
+    assert nbad_core == 0, f"{nbad_core} core flags differ"
+    assert nbad == 0, f"{nbad} labels differ"
+    return handle.last_stats
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_ex_txt(handle, golden, prec):
+    # test-datasets/ex.txt with the common.h defaults (eps 1.5, minpts 3, r 3)
+    lab = handle.run(EX, 1.5, 3, r=3, precision=prec)
+    assert lab.tolist() == golden["ex_labels"].tolist() == [1, 1, 1, 2, 2, 2, 3, 3, 3]
+    assert handle.core_flags(9).tolist() == [0, 1, 0, 0, 1, 0, 0, 1, 0]
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_cfg1_reference_labels(handle, golden, prec):
+    x, info = dclust.synth_config(1)
+    assert hashlib.sha256(x.tobytes()).digest() == golden["cfg1_sha256"].tobytes()
+    lab = handle.run(x, info["eps"], info["minpts"], r=info["partitions_r"], precision=prec)
+    assert np.array_equal(lab, golden["cfg1_labels"].astype(np.int32))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_lattice_ties_reference_labels(handle, golden, prec, k):
+    x = golden[f"lat{k}_x"]
+    eps, mp = golden[f"lat{k}_par"]
+    lab = handle.run(x, float(eps), int(mp), precision=prec)
+    assert np.array_equal(lab, golden[f"lat{k}_labels"].astype(np.int32))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_random_blobs(handle, prec, dim, seed):
+    rng = np.random.default_rng(100 + seed)
+    n = 20000
+    centres = rng.uniform(0, 1, (12, dim))
+    x = centres[rng.integers(0, 12, n)] + rng.normal(0, 0.02, (n, dim))
+    x[: n // 10] = rng.uniform(0, 1, (n // 10, dim))
+    eps = 0.01 if dim == 2 else 0.02
+    _check(handle, x, eps, 5, prec)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("seeds", [1, 2, 7])
+def test_tiny_seed_list_forces_correction(handle, prec, seeds):
+    rng = np.random.default_rng(7)
+    x = rng.normal(0, 1, (8000, 2))
+    st = _check(handle, x, 0.08, 4, prec, seeds=seeds)
+    assert st["n_overflow"] > 0 and st["n_passes"] > 1
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_tiny_pair_buffer_merges_inline(handle, prec):
+    rng = np.random.default_rng(8)
+    x = rng.normal(0, 1, (30000, 3))
+    st = _check(handle, x, 0.15, 6, prec, extra=1, blocks=64, seeds=16)
+    assert st["n_collisions"] > 64
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_cfg2_subset(handle, prec):
+    x, info = dclust.synth_config(2, n=200000)
+    _check(handle, x, info["eps"], info["minpts"], prec)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_cfg3_subset(handle, prec):
+    x, info = dclust.synth_config(3, n=400000)
+    _check(handle, x, info["eps"], info["minpts"], prec)
+
+
+def test_batched_matches_oracle(handle):
+    arrays, params, refs = [], [], []
+    for i in range(40):
+        x, info = dclust.synth_batch_instance(i)
+        if x.shape[0] > 30000:
+            continue
+        arrays.append(x)
+        params.append(dclust.make_params(x.shape[1], info["eps"], info["minpts"],
+                                         precision=dclust.FP32_GUARD))
+        refs.append(oracle.fast_dbscan(x, info["eps"], info["minpts"])[0])
+    labs, errs = handle.run_batched(arrays, params)
+    assert not np.any(errs)
+    for i, (a, b) in enumerate(zip(labs, refs)):
+        assert np.array_equal(a, b), f"instance {i}"

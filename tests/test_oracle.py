@@ -1,0 +1,185 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+- against the reference's own FP64 R-tree DBSCAN compiled in place
+  (oracle/_ref) and the committed golden labels it produced;
+- the fast formulation against the faithful sequential restatement (2-D and
+  3-D) -- the two share only the predicate;
+- core flags against an O(N^2) brute-force count;
+- the predicate against hand-built boundary pairs (closed box, exact eps^2
+  ties, FMA-sensitive sums) whose answers are fixed by IEEE arithmetic;
+- the cluster-check restatement against the reference checker compiled in
+  place, and against hand-computed Jaccard scores.
+"""
+import hashlib
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import dclust
+import oracle
+
+EX = np.array([[1, 1], [2, 2], [3, 3], [10, 10], [11, 11], [12, 12], [20, 20], [21, 21],
+               [22, 22]], dtype=np.float64)
+
+need_ref = pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+
+
+def test_ex_txt_known_answer(golden):
+    # common.h defaults: 9 points, minpts 3, eps 1.5 -> 3 clusters, 0 noise
+    lab = oracle.seq_dbscan(EX, 1.5, 3)
+    assert lab.tolist() == [1, 1, 1, 2, 2, 2, 3, 3, 3]
+    assert golden["ex_labels"].tolist() == lab.tolist()
+    f, core, asym = oracle.fast_dbscan(EX, 1.5, 3)
+    assert f.tolist() == lab.tolist() and core.tolist() == [0, 1, 0, 0, 1, 0, 0, 1, 0]
+
+
+def test_cfg1_golden(golden):
+    x, info = dclust.synth_config(1)
+    assert hashlib.sha256(x.tobytes()).digest() == golden["cfg1_sha256"].tobytes()
+    ref = golden["cfg1_labels"].astype(np.int32)
+    assert np.array_equal(oracle.seq_dbscan(x, info["eps"], info["minpts"]), ref)
+    assert np.array_equal(oracle.fast_dbscan(x, info["eps"], info["minpts"])[0], ref)
+
+
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_lattice_golden(golden, k):
+    x = golden[f"lat{k}_x"]
+    eps, mp = golden[f"lat{k}_par"]
+    ref = golden[f"lat{k}_labels"].astype(np.int32)
+    assert np.array_equal(oracle.seq_dbscan(x, float(eps), int(mp)), ref)
+    lab, _, asym = oracle.fast_dbscan(x, float(eps), int(mp))
+    assert asym == 0 and np.array_equal(lab, ref)
+
+
+@need_ref
+@pytest.mark.parametrize("seed", range(4))
+def test_seq_matches_compiled_reference_2d(seed):
+    rng = np.random.default_rng(seed)
+    n = 4000
+    c = rng.uniform(0, 10, (6, 2))
+    x = c[rng.integers(0, 6, n)] + rng.normal(0, 0.4, (n, 2))
+    for eps, mp in [(0.1, 3), (0.25, 6), (0.5, 12)]:
+        ref = oracle.ref_rtree_dbscan(x, eps, mp)
+        assert np.array_equal(oracle.seq_dbscan(x, eps, mp), ref)
+        assert np.array_equal(oracle.fast_dbscan(x, eps, mp)[0], ref)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("seed", range(3))
+def test_fast_matches_seq(dim, seed):
+    rng = np.random.default_rng(10 + seed)
+    n = 3000
+    c = rng.uniform(0, 1, (5, dim))
+    x = c[rng.integers(0, 5, n)] + rng.normal(0, 0.05, (n, dim))
+    x[:300] = rng.uniform(0, 1, (300, dim))
+    for eps, mp in [(0.02, 3), (0.05, 8)]:
+        s = oracle.seq_dbscan(x, eps, mp)
+        f, core, asym = oracle.fast_dbscan(x, eps, mp)
+        assert asym == 0
+        assert np.array_equal(s, f)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_core_flags_vs_brute_force(dim):
+    rng = np.random.default_rng(5)
+    x = rng.normal(0, 1, (2500, dim))
+    eps, mp = 0.2, 5
+    cnt = oracle.brute_counts(x, eps)
+    _, core, _ = oracle.fast_dbscan(x, eps, mp)
+    assert np.array_equal(core, (cnt >= mp).astype(np.uint8))
+    # a core point has a label; a point with count 1 (only itself) is noise
+    lab = oracle.seq_dbscan(x, eps, mp)
+    assert np.all(lab[cnt >= mp] > 0)
+
+
+def test_labels_are_components_brute_force():
+    # partition of core points == connected components of the core graph
+    rng = np.random.default_rng(6)
+    x = rng.normal(0, 1, (800, 2))
+    eps, mp = 0.15, 3
+    lab, core, _ = oracle.fast_dbscan(x, eps, mp)
+    n = x.shape[0]
+    adj = np.zeros((n, n), dtype=bool)
+    for i in range(n):
+        for j in range(n):
+            adj[i, j] = oracle.pred(x[i], x[j], eps)
+    comp = -np.ones(n, dtype=int)
+    k = 0
+    for s in range(n):
+        if core[s] and comp[s] < 0:
+            k += 1
+            stack = [s]
+            comp[s] = k
+            while stack:
+                u = stack.pop()
+                for v in np.nonzero(adj[u] & core.astype(bool))[0]:
+                    if comp[v] < 0:
+                        comp[v] = k
+                        stack.append(v)
+    # ids by smallest core index == component discovery order above
+    assert np.array_equal(lab[core == 1], comp[core == 1])
+    # borders: smallest adjacent component id, evaluated from the core side
+    for b in np.nonzero(core == 0)[0]:
+        ids = [comp[q] for q in range(n) if core[q] and adj[q, b]]
+        assert lab[b] == (min(ids) if ids else 0)
+
+
+def test_predicate_boundaries():
+    eps = 0.5
+    p = np.array([0.0, 0.0])
+    # exactly eps away on an axis: d^2 == eps^2 and box edge == q -> neighbour
+    assert oracle.pred(p, np.array([0.5, 0.0]), eps)
+    assert not oracle.pred(p, np.array([np.nextafter(0.5, 1.0), 0.0]), eps)
+    # closed box uses fl(p + eps): 0.1 + 0.2 rounds up past 0.3
+    assert oracle.pred(np.array([0.1, 0.0]), np.array([0.3, 0.0]), 0.2) == (0.3 <= 0.1 + 0.2)
+    # d^2 compared with fl(eps*eps), sum without FMA: (3,4) at eps 5 is a tie
+    assert oracle.pred(np.array([0.0, 0.0]), np.array([3.0, 4.0]), 5.0)
+    assert not oracle.pred(np.array([0.0, 0.0]), np.array([3.0, np.nextafter(4.0, 5.0)]), 5.0)
+    # 3-D left-to-right sum: 1+4+4 = 9 = 3^2
+    assert oracle.pred(np.zeros(3), np.array([1.0, 2.0, 2.0]), 3.0)
+    # a point is its own neighbour
+    assert oracle.pred(np.array([1e6, -3.0]), np.array([1e6, -3.0]), 1e-9)
+
+
+def test_predicate_no_fma_contraction():
+    # chosen so that fl(fl(a*a) + fl(b*b)) <= fl(e*e) but the fused form is not
+    rng = np.random.default_rng(9)
+    found = 0
+    for _ in range(200000):
+        a, b = rng.uniform(0.1, 1.0, 2)
+        e = np.sqrt(a * a + b * b)
+        sep = np.float64(a * a) + np.float64(b * b)
+        fused = np.longdouble(a) * np.longdouble(a) + np.longdouble(b) * np.longdouble(b)
+        if (sep <= e * e) != (np.float64(fused) <= e * e):
+            found += 1
+            assert oracle.pred(np.array([0.0, 0.0]), np.array([a, b]), e) == bool(sep <= e * e)
+    assert found > 0
+
+
+@need_ref
+def test_cluster_check_matches_reference_tool(tmp_path):
+    rng = np.random.default_rng(3)
+    n = 3000
+    a = rng.integers(0, 8, n).astype(np.int32)
+    b = a.copy()
+    flip = rng.random(n) < 0.1
+    b[flip] = rng.integers(0, 8, flip.sum())
+    pa, pb = tmp_path / "a.txt", tmp_path / "b.txt"
+    np.savetxt(pa, a, fmt="%d")
+    np.savetxt(pb, b, fmt="%d")
+    out = subprocess.run([oracle.REF_CHECK, str(pa), str(pb)], capture_output=True, text=True,
+                         check=True).stdout
+    ne, eq, lost, score = oracle.cluster_check(a, b)
+    assert f"mismatched noise points: {ne}, agreement noise points: {eq}" in out
+    assert f"fraction lost due to mismatches between clusters: {lost:f}" in out
+    assert f"Final Error metric: {score:f}" in out
+
+
+def test_cluster_check_hand_values():
+    assert oracle.cluster_check([1, 1, 2, 2], [1, 1, 2, 2]) == (0, 0, 0.0, 1.0)
+    # point 0,1 in cluster {0,1} vs {0,1,2}: J = 2/3; point 2: {2,3} vs {0,1,2}: 1/4
+    ne, eq, lost, score = oracle.cluster_check([1, 1, 2, 2], [1, 1, 1, 0])
+    assert ne == 1 and eq == 0
+    assert score == pytest.approx((2 / 3 + 2 / 3 + 1 / 4 + 0) / 4)
