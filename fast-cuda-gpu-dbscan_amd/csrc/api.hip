@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -32,7 +33,7 @@ struct Buf {
   size_t cap = 0;
 };
 
-enum Ev { E_START, E_INDEX, E_CORE0, E_CORE, E_EXPAND, E_MERGE, E_BORDER, E_END, E_N };
+enum Ev { E_START, E_INDEX, E_CORE0, E_CORE, E_PASS0, E_EXPAND, E_MERGE, E_BORDER, E_END, E_N };
 
 }  // namespace
 
@@ -79,8 +80,12 @@ enum Slot {
   S_XYZ, S_INST, S_CHUNK, S_BBOX, S_CTL, S_KEY0, S_KEY1, S_VAL0, S_VAL1, S_HIST, S_OFFS,
   S_DTOT, S_PASS, S_CSTART, S_CKEY, S_CINST, S_CM, S_ROWS, S_CELLOF, S_X0, S_X1, S_X2,
   S_G0, S_G1, S_G2, S_ORIG, S_CORE, S_LABEL, S_PARENT, S_COMPMIN, S_PAIRS, S_Q0, S_Q1,
-  S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT
+  S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT, S_PAIRCNT, S_SEGPOS, S_LINES
 };
+
+// line table entries (8 bytes each) above which neighbour lines are found
+// by binary search instead
+constexpr long long kMaxLineTable = 1LL << 25;
 
 int check_params(const dclust_params* p) {
   if (!p) return DCLUST_E_PARAM;
@@ -113,6 +118,9 @@ struct RunShape {
   int seeds;
   int threads;
   int blocks;
+  int waves;
+  int pair_per;
+  int budget;
   int64_t pair_cap;
 };
 
@@ -129,8 +137,16 @@ RunShape pick_shape(dclust_handle* h, const dclust_params& p, int64_t n, bool f3
     r.blocks = h->num_cu * std::min(per_cu, 8);
   }
   r.blocks = std::max(8, (r.blocks + 7) / 8 * 8);  // multiple of 8 (XCD remap)
-  int64_t per = p.extra_collision > 0 ? p.extra_collision : 0;
-  r.pair_cap = per > 0 ? per * r.blocks : std::max<int64_t>(1 << 16, n / 4);
+  // EXTRA_COLLISION_SIZE: collision pairs buffered per workgroup; split over
+  // its wavefronts (each wavefront is a chain owner)
+  const int waves_per_block = r.threads / 64;
+  r.waves = r.blocks * waves_per_block;
+  r.pair_per = p.extra_collision > 0 ? std::max(1, p.extra_collision / waves_per_block) : 512;
+  r.pair_cap = (int64_t)r.pair_per * r.waves;
+  // seed batches a wave expands per pass before handing its list on
+  r.budget = 256;
+  if (const char* e = std::getenv("DCLUST_EXPAND_BUDGET")) r.budget = std::max(1, std::atoi(e));
+  (void)n;
   return r;
 }
 
@@ -182,7 +198,8 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     ntot += n[i];
     off += n[i] * I.dim;
   }
-  if (ntot >= (int64_t)INT32_MAX - 1) return DCLUST_E_CAPACITY;
+  // cell indices are packed into 28 bits in the neighbour-line table
+  if (ntot >= kMaxCells) return DCLUST_E_CAPACITY;
   if (inst_err)
     for (int32_t i = 0; i < ninst; ++i) inst_err[i] = 0;
   if (stats) stats->n_points = ntot;
@@ -200,6 +217,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
   d.nchunks = nchunks;
   d.xyz = d_xyz;
   d.pair_cap = shape.pair_cap;
+  d.pair_per = shape.pair_per;
   {
     hipError_t& E = herr;
     d.inst = h->get<Inst>(S_INST, ninst, &E);
@@ -217,8 +235,8 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     d.cell_start = h->get<int32_t>(S_CSTART, ntot + 1, &E);
     d.cell_key = h->get<uint64_t>(S_CKEY, ntot, &E);
     d.cell_inst = h->get<int32_t>(S_CINST, ntot, &E);
-    d.cell_m = h->get<int32_t>(S_CM, ntot, &E);
-    d.rows = h->get<int2>(S_ROWS, (size_t)ntot * kMaxRows, &E);
+    d.cell_ncore = h->get<int32_t>(S_CM, ntot, &E);
+    d.rows = h->get<int32_t>(S_ROWS, (size_t)ntot * kMaxRows, &E);
     d.cell_of = h->get<int32_t>(S_CELLOF, ntot, &E);
     d.X[0] = h->get<double>(S_X0, ntot, &E);
     d.X[1] = h->get<double>(S_X1, ntot, &E);
@@ -228,10 +246,12 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     d.G[2] = h->get<uint32_t>(S_G2, ntot, &E);
     d.orig = h->get<int32_t>(S_ORIG, ntot, &E);
     d.core = h->get<uint8_t>(S_CORE, ntot, &E);
-    d.label = h->get<int32_t>(S_LABEL, ntot, &E);
+    d.cell_label = h->get<int32_t>(S_LABEL, ntot, &E);
     d.parent = h->get<int32_t>(S_PARENT, ntot, &E);
     d.compmin = h->get<int32_t>(S_COMPMIN, ntot, &E);
     d.pairs = h->get<int2>(S_PAIRS, shape.pair_cap, &E);
+    d.pair_cnt = h->get<int32_t>(S_PAIRCNT, shape.waves, &E);
+    d.seg_pos = h->get<int64_t>(S_SEGPOS, shape.waves, &E);
     d.q[0] = h->get<int32_t>(S_Q0, ntot, &E);
     d.q[1] = h->get<int32_t>(S_Q1, ntot, &E);
     d.keyv = h->get<int32_t>(S_KEYV, ntot, &E);
@@ -245,6 +265,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
 
   {
     long long ctl_h[C_NCTL];
+    float t_pass0_ms = 0.f;
     int passes = 0;
     int npass = 0;
     HIPCHK(hipMemcpyAsync(d.inst, inst.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, s));
@@ -269,26 +290,38 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     }
     passes = (int)((ctl_h[C_KEYBITS] + 7) / 8);
     if (stats) stats->key_bits = (int32_t)ctl_h[C_KEYBITS];
+    // dense line table when it is small enough, else binary search
+    d.line_cap = 0;
+    if (ctl_h[C_NLINES] <= kMaxLineTable) {
+      d.lines = h->get<int2>(S_LINES, (size_t)std::max<long long>(ctl_h[C_NLINES], 1), &herr);
+      if (!d.lines) return DCLUST_E_OOM;
+      d.line_cap = ctl_h[C_NLINES];
+    }
     launch_index_tail(d, ntot, nchunks, grid_cells, passes, s);
     HIPCHK(hipEventRecord(h->ev[E_INDEX], s));
 
-    launch_fill_i32(d.label, ntot, kUnclaimed, s);
+    launch_fill_i32(d.cell_label, ntot, kUnclaimed, s);
     launch_iota(d.parent, ntot, s);
     launch_fill_i32(d.compmin, ntot, kNoKey, s);
     launch_fill_u8(d.flag, ntot, 0, s);
+    launch_fill_i32(d.pair_cnt, shape.waves, 0, s);
     // the core (neighbour-count) kernel is bracketed by its own events
     HIPCHK(hipEventRecord(h->ev[E_CORE0], s));
     launch_core(d, f32, grid_cells, s);
     HIPCHK(hipEventRecord(h->ev[E_CORE], s));
 
+    // expansion passes until no seed is pending: pass p drains the queue of
+    // pass p-1 (overflowed seed lists, lists left when a wave's pop budget
+    // ran out) and continues every unfinished seeding segment
+    launch_seg_init(d, shape.waves, s);
     for (npass = 0; npass < kMaxExpandPasses; ++npass) {
       launch_pass_prep(d, npass, s);
-      launch_expand(d, f32, npass, shape.blocks, shape.seeds, shape.threads, s);
-      long long qn = 0;
-      HIPCHK(hipMemcpyAsync(&qn, d.ctl + C_Q0_N + ((npass + 1) & 1), sizeof(qn),
-                            hipMemcpyDeviceToHost, s));
+      launch_expand(d, f32, npass, shape.blocks, shape.seeds, shape.threads, shape.budget, s);
+      launch_pass_carry(d, npass, s);
+      if (npass == 0) HIPCHK(hipEventRecord(h->ev[E_PASS0], s));
+      HIPCHK(hipMemcpyAsync(ctl_h, d.ctl, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
-      if (qn == 0) break;
+      if (ctl_h[C_Q0_N + ((npass + 1) & 1)] == 0 && ctl_h[C_SEGS_LEFT] == 0) break;
     }
     ++npass;
     HIPCHK(hipEventRecord(h->ev[E_EXPAND], s));
@@ -301,6 +334,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ctl_h, d.ctl, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    t_pass0_ms = ev_ms(h, E_CORE, E_PASS0);
     h->last_n = ntot;
     h->last_core = d.core_out;
     if (stats) {
@@ -325,6 +359,16 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
       stats->band_flips = ctl_h[C_FLIP];
       stats->asym_pairs = ctl_h[C_ASYM];
       stats->pairs_evaluated = ctl_h[C_EVAL];
+      stats->band_rechecks_core = ctl_h[C_BAND_CORE];
+      stats->core_cand_loads = ctl_h[C_CANDLOAD];
+      stats->diag[0] = ctl_h[C_POPS];
+      stats->diag[1] = ctl_h[C_DENSE];
+      stats->diag[2] = ctl_h[C_XCHUNK];
+      stats->diag[3] = ctl_h[C_XREL];
+      stats->diag[4] = (int64_t)(1000.0 * t_pass0_ms);
+      stats->diag[5] = ctl_h[C_BUSY_SUM];
+      stats->diag[6] = ctl_h[C_BUSY_MAX];
+      stats->diag[7] = shape.waves;
     }
     if (npass >= kMaxExpandPasses) return DCLUST_E_CAPACITY;
     return rc;
@@ -501,7 +545,7 @@ int dclust_autotune(dclust_handle* h, int64_t n, dclust_params* p) {
   if (p->seed_list_size == 0) p->seed_list_size = r.seeds;
   if (p->block_threads == 0) p->block_threads = r.threads;
   if (p->blocks == 0) p->blocks = r.blocks;
-  if (p->extra_collision == 0) p->extra_collision = (int32_t)std::min<int64_t>(INT32_MAX, r.pair_cap / r.blocks);
+  if (p->extra_collision == 0) p->extra_collision = r.pair_per * (r.threads / 64);
   return DCLUST_OK;
 }
 

@@ -131,34 +131,48 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d) {
 }
 
 // ------------------------------------------------------------ grid setup --
-// Cells are cubes of width w: w = max(min extent / r, eps*(1+2^-20)) when r
-// is given (DCLUST_E_BINWIDTH if min extent / r < eps, as main.cu:204-207
-// refuses), w = eps*(1+2^-20) otherwise.  The 2^-20 margin keeps every
-// eps-box inside the 3^D cells around its centre despite rounding of the
-// cell function (the rounding error of floor(fl(fl(x-mn)*inv_w)) is below
-// 5*2^-53 * cells-per-axis).  Axes are capped at 2^21 cells (3-D) or 2^31
-// (2-D) by widening w, which only coarsens the grid.
+// Cells are cubes of width w = (eps/2)*(1+2^-20).  Two consequences:
+//  * every cell is a clique: its diagonal sqrt(D)*w <= 0.87*eps, so any two
+//    of its points are neighbours with a wide margin (FP64 rounding of the
+//    box test and of d^2 is far below that margin);
+//  * every eps-neighbour of a point lies in the 5^D cells around its cell
+//    (|dx| <= eps < 2w, and the cell function is monotone, so the
+//    rounding of floor(fl(fl(x-mn)*inv_w)) -- below 5*2^-53 per cell --
+//    cannot move a neighbour further than the 2^-20 margin allows).
+// PARTITION_SIZE r is validated as the reference does (min extent / r must
+// be >= eps, main.cu:204-207, else DCLUST_E_BINWIDTH); it does not change
+// the grid, whose resolution is fixed by eps (DBSCAN labels do not depend on
+// the index).  Axes are limited to 2^21 cells (3-D) / 2^31 (2-D), i.e. an
+// extent of about 2^20 eps; larger inputs get DCLUST_E_CAPACITY.
 //
-// FP32 form: distances are formed from fixed-point cell offsets in units of
-// w*2^-30.  The FP32 squared distance D2 differs from the exact squared
-// distance (same units) by at most 5*2^-24 relative (int->float conversion,
-// square, two fused adds) plus the quantisation of the offsets.  Pairs with
-// |D2 - eps^2| <= band are re-decided with the exact FP64 predicate; band is
-// 8 ulp = 2^-20 relative unless the bound above (plus the box-test slack
-// from coordinate magnitude) requires more, in which case it is widened to
-// twice the bound.
+// FP32 form: distances are formed from fixed-point coordinates in units of
+// w*2^-29 (29 fraction bits, wrapping every 8 cells; stencil differences
+// stay below 3 cells < 2^31 units).  The FP32 squared distance D2 differs
+// from the exact squared distance (same units) by at most 5*2^-24 relative
+// (int->float conversion, square, two fused adds) plus the quantisation of
+// the coordinates.  Pairs with |D2 - eps^2| <= band are re-decided with the
+// exact FP64 predicate; band is 8 ulp = 2^-20 relative unless the bound
+// above (plus the box-test slack from coordinate magnitude) requires more,
+// in which case it is widened to twice the bound.
+//
+// Per instance this also sets the line-table base: a line is the set of
+// cells sharing all coordinates but the last; the dense line table (one
+// entry per possible line) locates a neighbour line in O(1).
 __global__ void k_grid_setup(Dev d) {
   __shared__ long long s_scan[1024];
-  __shared__ long long s_carry;
+  __shared__ long long s_scanl[1024];
+  __shared__ long long s_carry, s_carryl;
   __shared__ int s_err;
   if (threadIdx.x == 0) {
     s_carry = 0;
+    s_carryl = 0;
     s_err = 0;
   }
   __syncthreads();
   for (int base = 0; base < d.ninst; base += blockDim.x) {
     const int i = base + threadIdx.x;
     long long cells = 0;
+    long long lines = 0;
     if (i < d.ninst) {
       Inst I = d.inst[i];
       const unsigned long long* bb = d.bbox + (int64_t)i * 8;
@@ -173,25 +187,29 @@ __global__ void k_grid_setup(Dev d) {
       }
       I.maxabs = empty ? 0.0 : unord_d(bb[6]);
       if (bb[7]) I.err = -1;  // DCLUST_E_PARAM: non-finite input
-      const double wmin = I.eps * (1.0 + 0x1p-20);
-      double w = wmin;
       if (I.r > 0) {
         double wr = emin / (double)I.r;
         if (!(wr >= I.eps) && I.err == 0 && !empty) I.err = -2;  // DCLUST_E_BINWIDTH
-        if (wr > w) w = wr;
       }
+      // coordinates so large that FP64 cannot resolve eps
+      if (I.maxabs > I.eps * 0x1p40 && I.err == 0) I.err = -1;
+      double w = 0.5 * I.eps * (1.0 + 0x1p-20);
       const double cap = dim == 3 ? 2097152.0 : 2147483647.0;
       for (;;) {
+        // an oversized input keeps a valid (coarser) grid so that every
+        // kernel stays in bounds; its result is discarded (error set)
         double inv = 1.0 / w;
         bool ok = true;
         for (int k = 0; k < dim; ++k)
           if (floor((mx[k] - mn[k]) * inv) + 1.0 > cap) ok = false;
         if (ok) break;
+        if (I.err == 0) I.err = -6;  // DCLUST_E_CAPACITY
         w *= 2.0;
       }
       I.w = w;
       I.inv_w = 1.0 / w;
       cells = 1;
+      lines = 1;
       for (int k = 0; k < 3; ++k) {
         I.mn[k] = k < dim ? mn[k] : 0.0;
         I.nk[k] = 1;
@@ -199,10 +217,12 @@ __global__ void k_grid_setup(Dev d) {
           double t = floor(cell_t(mx[k], mn[k], I.inv_w));
           I.nk[k] = (int32_t)t + 1;
           cells *= (long long)I.nk[k];
+          if (k < dim - 1) lines *= (long long)I.nk[k];
         }
       }
-      // FP32 guard band (units of w * 2^-30)
-      const double gs = I.inv_w * 0x1p30;
+      I.nlines = lines;
+      // FP32 guard band (units of w * 2^-29)
+      const double gs = I.inv_w * 0x1p29;
       const double es = I.eps * gs;
       const double e2 = es * es;
       const double u = 0x1p-24;
@@ -222,17 +242,26 @@ __global__ void k_grid_setup(Dev d) {
       if (I.err) atomicOr(&s_err, 1);
     }
     s_scan[threadIdx.x] = cells;
+    s_scanl[threadIdx.x] = lines;
     __syncthreads();
-    // inclusive Hillis-Steele scan over the block
+    // inclusive Hillis-Steele scans over the block
     for (int o = 1; o < (int)blockDim.x; o <<= 1) {
       long long v = threadIdx.x >= (unsigned)o ? s_scan[threadIdx.x - o] : 0;
+      long long vl = threadIdx.x >= (unsigned)o ? s_scanl[threadIdx.x - o] : 0;
       __syncthreads();
       s_scan[threadIdx.x] += v;
+      s_scanl[threadIdx.x] += vl;
       __syncthreads();
     }
-    if (i < d.ninst) d.inst[i].key_base = s_carry + s_scan[threadIdx.x] - cells;
+    if (i < d.ninst) {
+      d.inst[i].key_base = s_carry + s_scan[threadIdx.x] - cells;
+      d.inst[i].line_base = s_carryl + s_scanl[threadIdx.x] - lines;
+    }
     __syncthreads();
-    if (threadIdx.x == blockDim.x - 1) s_carry += s_scan[threadIdx.x];
+    if (threadIdx.x == blockDim.x - 1) {
+      s_carry += s_scan[threadIdx.x];
+      s_carryl += s_scanl[threadIdx.x];
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -241,6 +270,7 @@ __global__ void k_grid_setup(Dev d) {
     while (bits < 63 && (1LL << bits) < total) ++bits;
     d.ctl[C_KEYBITS] = bits;
     d.ctl[C_ERR] = s_err;
+    d.ctl[C_NLINES] = s_carryl;
   }
 }
 
@@ -556,8 +586,8 @@ __global__ __launch_bounds__(256) void k_gather(Dev d) {
       double x = p[k];
       d.X[k][i] = x;
       double t = cell_t(x, I.mn[k], I.inv_w);       // same rounding as the key
-      double f = floor(t * 0x1p30);                  // exact scaling by 2^30
-      d.G[k][i] = (uint32_t)(unsigned long long)(long long)f;  // wraps mod 4 cells
+      double f = floor(t * 0x1p29);                  // exact scaling by 2^29
+      d.G[k][i] = (uint32_t)(unsigned long long)(long long)f;  // wraps mod 8 cells
     }
     d.orig[i] = src;
   }
@@ -573,62 +603,111 @@ __device__ __forceinline__ int32_t lower_bound_key(const uint64_t* a, int32_t n,
   return lo;
 }
 
-// For each cell: the 3^(D-1) rows of neighbour cells (all axes but the last
-// fixed at offsets -1/0/+1, the last axis spanning k-1..k+1), as contiguous
-// point ranges.  Row 0 is the cell's own row.
-__global__ __launch_bounds__(256) void k_rows(Dev d) {
+__device__ __forceinline__ int32_t lower_bound_key_in(const uint64_t* a, int32_t lo, int32_t hi,
+                                                      uint64_t v) {
+  while (lo < hi) {
+    int32_t mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// line of a cell: all coordinates but the last; global line-table index
+__device__ __forceinline__ int64_t cell_line(const Dev& d, int32_t c) {
+  const Inst& I = d.inst[d.cell_inst[c]];
+  const int64_t lin = (int64_t)(d.cell_key[c] - (uint64_t)I.key_base);
+  return I.line_base + lin / I.nk[I.dim - 1];
+}
+
+// Line table: for every possible line, the range of its non-empty cells in
+// the sorted cell array (-1 when empty).
+__global__ __launch_bounds__(256) void k_line_clear(Dev d) {
+  const long long P = d.ctl[C_NLINES];
+  if (P > d.line_cap) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d.lines[i] = make_int2(-1, -1);
+}
+
+__global__ __launch_bounds__(256) void k_line_mark(Dev d) {
   const int32_t C = (int32_t)d.ctl[C_NCELL];
-  long long cand = 0, nbrc = 0;
+  if (d.ctl[C_NLINES] > d.line_cap) return;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < C;
        c += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t L = cell_line(d, (int32_t)c);
+    if (c == 0 || cell_line(d, (int32_t)c - 1) != L) d.lines[L].x = (int32_t)c;
+    if (c == C - 1 || cell_line(d, (int32_t)c + 1) != L) d.lines[L].y = (int32_t)c + 1;
+  }
+}
+
+// Neighbour lines of the 5^D stencil, nearest first: (0,0), the 4 at
+// distance 1, the 4 diagonals, then the ring at distance 2.
+__constant__ int8_t kRowOff3[25][2] = {
+    {0, 0},   {-1, 0}, {1, 0},  {0, -1}, {0, 1},  {-1, -1}, {-1, 1}, {1, -1}, {1, 1},
+    {-2, 0},  {2, 0},  {0, -2}, {0, 2},  {-2, -1}, {-2, 1}, {2, -1}, {2, 1},  {-1, -2},
+    {1, -2},  {-1, 2}, {1, 2},  {-2, -2}, {-2, 2}, {2, -2}, {2, 2}};
+__constant__ int8_t kRowOff2[5] = {0, -1, 1, -2, 2};
+
+// For each (cell, stencil line): the run of non-empty cells of that line
+// whose last coordinate is within +-2 of the cell's, packed as
+// first | count << 28 (count <= 5).  One thread per (cell, line).  The
+// neighbour line is found in the dense line table, or by binary search over
+// all cells when the table would be too large.
+__global__ __launch_bounds__(256) void k_rows(Dev d) {
+  const int32_t C = (int32_t)d.ctl[C_NCELL];
+  const bool table = d.ctl[C_NLINES] <= d.line_cap;
+  long long cand = 0, nbrc = 0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)C * kMaxRows;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t c = (int32_t)(t / kMaxRows);
+    const int r = (int)(t - (int64_t)c * kMaxRows);
     const Inst& I = d.inst[d.cell_inst[c]];
     const int dim = I.dim;
-    int64_t lin = (int64_t)(d.cell_key[c] - (uint64_t)I.key_base);
-    int32_t kc[3];
-    for (int k = dim - 1; k >= 0; --k) {
-      kc[k] = (int32_t)(lin % I.nk[k]);
-      lin /= I.nk[k];
-    }
-    const int last = dim - 1;
-    const int nrow = dim == 3 ? 9 : 3;
-    const int32_t nc = d.cell_start[c + 1] - d.cell_start[c];
-    int32_t m = 0, kcnt = 0;
-    for (int r = 0; r < nrow; ++r) {
-      // row order: centre first, then the others
+    const int nrow = dim == 3 ? 25 : 5;
+    int32_t packed = 0;
+    if (r < nrow) {
+      int64_t lin = (int64_t)(d.cell_key[c] - (uint64_t)I.key_base);
+      int32_t kc[3] = {0, 0, 0};
+      for (int k = dim - 1; k >= 0; --k) {
+        kc[k] = (int32_t)(lin % I.nk[k]);
+        lin /= I.nk[k];
+      }
+      const int last = dim - 1;
       int o0, o1 = 0;
       if (dim == 3) {
-        static constexpr int8_t off[9][2] = {{0, 0}, {-1, -1}, {-1, 0}, {-1, 1}, {0, -1},
-                                             {0, 1},  {1, -1},  {1, 0},  {1, 1}};
-        o0 = off[r][0];
-        o1 = off[r][1];
+        o0 = kRowOff3[r][0];
+        o1 = kRowOff3[r][1];
       } else {
-        static constexpr int8_t off2[3] = {0, -1, 1};
-        o0 = off2[r];
+        o0 = kRowOff2[r];
       }
-      int32_t rc[3] = {kc[0] + o0, dim == 3 ? kc[1] + o1 : kc[1], dim == 3 ? kc[2] : 0};
-      bool ok = rc[0] >= 0 && rc[0] < I.nk[0];
-      if (dim == 3) ok = ok && rc[1] >= 0 && rc[1] < I.nk[1];
-      int2 range = make_int2(0, 0);
+      const int32_t r0 = kc[0] + o0, r1 = kc[1] + o1;
+      bool ok = r0 >= 0 && r0 < I.nk[0];
+      if (dim == 3) ok = ok && r1 >= 0 && r1 < I.nk[1];
       if (ok) {
-        int32_t lo_l = kc[last] > 0 ? kc[last] - 1 : 0;
-        int32_t hi_l = kc[last] + 1 < I.nk[last] ? kc[last] + 1 : I.nk[last] - 1;
-        int64_t base = rc[0];
-        if (dim == 3) base = base * I.nk[1] + rc[1];
-        base *= I.nk[last];
-        uint64_t klo = (uint64_t)(I.key_base + base + lo_l);
-        uint64_t khi = (uint64_t)(I.key_base + base + hi_l);
-        int32_t a = lower_bound_key(d.cell_key, C, klo);
-        int32_t b = a;
-        while (b < C && d.cell_key[b] <= khi) ++b;
-        range = make_int2(d.cell_start[a], d.cell_start[b]);
-        kcnt += b - a;
+        const int32_t lo_l = kc[last] >= 2 ? kc[last] - 2 : 0;
+        const int32_t hi_l = kc[last] + 2 < I.nk[last] ? kc[last] + 2 : I.nk[last] - 1;
+        const int64_t line = dim == 3 ? (int64_t)r0 * I.nk[1] + r1 : (int64_t)r0;
+        const int64_t base = line * I.nk[last];
+        const uint64_t klo = (uint64_t)(I.key_base + base + lo_l);
+        const uint64_t khi = (uint64_t)(I.key_base + base + hi_l);
+        int32_t ls = 0, le = C;
+        if (table) {
+          const int2 lr = d.lines[I.line_base + line];
+          ls = lr.x;
+          le = lr.y;
+        }
+        if (ls >= 0) {
+          const int32_t a = lower_bound_key_in(d.cell_key, ls, le, klo);
+          int32_t b = a;
+          while (b < le && d.cell_key[b] <= khi) ++b;
+          packed = (b > a) ? (a | ((b - a) << kRowCntShift)) : 0;
+          nbrc += b - a;
+          cand += (long long)(d.cell_start[c + 1] - d.cell_start[c]) *
+                  (d.cell_start[b] - d.cell_start[a]);
+        }
       }
-      d.rows[c * kMaxRows + r] = range;
-      m += range.y - range.x;
     }
-    d.cell_m[c] = m;
-    cand += (long long)nc * m;
-    nbrc += kcnt;
+    d.rows[t] = packed;
   }
   wave_add(&d.ctl[C_CAND], cand);
   wave_add(&d.ctl[C_NBRC], nbrc);
@@ -656,7 +735,9 @@ void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, d.scan_tmp, ncs, (long long*)nullptr);
   hipLaunchKernelGGL(k_csr_write, dim3(ncs), dim3(CS_T), 0, s, d);
   hipLaunchKernelGGL(k_gather, dim3(fill_grid(n)), dim3(256), 0, s, d);
-  hipLaunchKernelGGL(k_rows, dim3(grid_cells), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_line_clear, dim3(grid_cells), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_line_mark, dim3(grid_cells), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_rows, dim3(grid_cells * 4), dim3(256), 0, s, d);
 }
 
 void launch_scan_top(int64_t* v, int nb, hipStream_t s) {

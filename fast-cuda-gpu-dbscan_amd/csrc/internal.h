@@ -21,8 +21,11 @@ namespace dclust {
 constexpr int kWave = 64;
 constexpr int32_t kUnclaimed = -1;
 constexpr int32_t kNoKey = 0x7fffffff;
-constexpr int kFixBits = 30;  // fixed-point fraction bits per cell width
-constexpr int kMaxRows = 9;   // 3^(D-1) neighbour rows for D <= 3
+constexpr int kFixBits = 29;  // fixed-point fraction bits per cell width
+constexpr int kMaxRows = 25;  // 5^(D-1) neighbour lines for D <= 3
+constexpr int kRowCntShift = 28;  // packed row: first cell | cell count << 28
+constexpr int32_t kRowCellMask = (1 << kRowCntShift) - 1;
+constexpr int64_t kMaxCells = 1LL << kRowCntShift;
 
 // Per-instance parameters and derived grid (device table, one per instance).
 struct Inst {
@@ -41,6 +44,8 @@ struct Inst {
   int32_t nk[3];   // cells per axis
   int32_t err;     // per-instance error code (DCLUST_E_*)
   int64_t key_base;            // first key of this instance's cell space
+  int64_t line_base;           // first entry of this instance's line table
+  int64_t nlines;              // possible lines (product of all but the last axis)
   int64_t pt_begin, pt_end;    // global point-index range
   int64_t xyz_off;             // element offset of the first coordinate
 };
@@ -75,6 +80,17 @@ enum Ctl : int {
   C_OVF,
   C_ERR,
   C_SORTPAR,  // number of executed radix scatters (buffer parity)
+  C_BAND_CORE,
+  C_CANDLOAD,
+  C_POPS,
+  C_BATCH,
+  C_XCHUNK,
+  C_XREL,
+  C_BUSY_SUM,  // expansion: sum over waves of their lifetime (us)
+  C_BUSY_MAX,
+  C_SEGS_LEFT,  // expansion waves whose seeding segment is not finished
+  C_NLINES,     // total line-table entries
+  C_DENSE,      // cells resolved as all-core without distance tests
   C_NCTL = 32
 };
 
@@ -106,9 +122,12 @@ struct Dev {
   int32_t* cell_start;  // [N+1]
   uint64_t* cell_key;   // [N]
   int32_t* cell_inst;   // [N]
-  int32_t* cell_m;      // [N] candidate points in the neighbourhood
-  int2* rows;           // [N][kMaxRows]
+  int32_t* cell_ncore;  // [N] core points per cell
+  int32_t* cell_label;  // [N] chain owning the cell's core points
+  int32_t* rows;        // [N][kMaxRows] packed neighbour lines (first cell, count)
   int32_t* cell_of;     // [N] per sorted point
+  int2* lines;          // [line_cap] first / end cell of every possible line
+  int64_t line_cap;     // 0: no line table, rows found by binary search
 
   // sorted point data
   double* X[3];
@@ -118,9 +137,12 @@ struct Dev {
   int32_t* label;
   int32_t* parent;
   int32_t* compmin;
-  int2* pairs;
-  int64_t pair_cap;
+  int2* pairs;          // [waves][pair_per] collision pairs per expansion wave
+  int32_t* pair_cnt;    // [waves] pairs recorded (may exceed pair_per)
+  int64_t pair_cap;     // waves * pair_per
+  int32_t pair_per;
   int32_t* q[2];
+  int64_t* seg_pos;     // [waves] seeding cursor of each expansion wave
   int32_t* keyv;
   uint8_t* flag;
   int32_t* rank;
@@ -237,6 +259,13 @@ __device__ __forceinline__ double cell_t(double x, double mn, double inv_w) {
 }
 
 // ---- chain union-find (lock-free; roots are the minimum id) ------------
+// Invariant: par[x] <= x, and a link only ever replaces a root's self-pointer
+// (by CAS) with a smaller id, so every value par[x] has held is an ancestor
+// of x.  A plain load may therefore return a stale but still valid ancestor:
+// uf_find_cached walks with plain (L2-cached) loads, which is enough to
+// decide "same set" when two walks meet.  A CAS on a stale "root" fails and
+// the retry walks with coherent (agent-scope) loads, which guarantees
+// progress.
 __device__ __forceinline__ int32_t uf_find(int32_t* par, int32_t x) {
   for (;;) {
     int32_t p = __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -248,17 +277,30 @@ __device__ __forceinline__ int32_t uf_find(int32_t* par, int32_t x) {
   }
 }
 
-__device__ __forceinline__ void uf_union(int32_t* par, int32_t a, int32_t b) {
+__device__ __forceinline__ int32_t uf_find_cached(int32_t* par, int32_t x) {
   for (;;) {
-    a = uf_find(par, a);
-    b = uf_find(par, b);
+    int32_t p = par[x];
+    if (p == x) return x;
+    int32_t gp = par[p];
+    if (gp != p) par[x] = gp;  // path halving: gp is an ancestor of x
+    x = p;
+  }
+}
+
+__device__ __forceinline__ void uf_union(int32_t* par, int32_t a, int32_t b) {
+  a = uf_find_cached(par, a);
+  b = uf_find_cached(par, b);
+  for (;;) {
     if (a == b) return;
     if (a < b) {
       int32_t t = a;
       a = b;
       b = t;
     }
-    if (atomicCAS(&par[a], a, b) == a) return;
+    int32_t old = atomicCAS(&par[a], a, b);
+    if (old == a) return;
+    a = uf_find(par, old);
+    b = uf_find(par, b);
   }
 }
 

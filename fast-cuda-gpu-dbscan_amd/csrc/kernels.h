@@ -24,8 +24,10 @@ int64_t scan_tmp_entries(int64_t n);
 
 void launch_core(const Dev& d, bool f32, int grid, hipStream_t s);
 void launch_pass_prep(const Dev& d, int pass, hipStream_t s);
+void launch_pass_carry(const Dev& d, int pass, hipStream_t s);
+void launch_seg_init(const Dev& d, int64_t waves, hipStream_t s);
 void launch_expand(const Dev& d, bool f32, int pass, int grid, int seeds,
-                   int threads, hipStream_t s);
+                   int threads, int budget, hipStream_t s);
 void launch_merge(const Dev& d, int grid, hipStream_t s);
 void launch_border(const Dev& d, bool f32, int grid, int64_t n, hipStream_t s);
 void launch_relabel(const Dev& d, int64_t n, hipStream_t s);

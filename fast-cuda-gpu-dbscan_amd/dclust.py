@@ -55,16 +55,19 @@ _STAT_F64 = ["t_total_ms", "t_index_ms", "t_core_ms", "t_expand_ms", "t_merge_ms
              "t_border_ms", "t_relabel_ms"]
 _STAT_I64 = ["n_points", "n_cells", "n_core", "n_noise", "n_clusters", "n_chains",
              "n_collisions", "n_overflow", "n_passes", "cand_total", "nbr_cells_total",
-             "band_rechecks", "band_flips", "asym_pairs", "pairs_evaluated"]
+             "band_rechecks", "band_flips", "asym_pairs", "pairs_evaluated",
+             "band_rechecks_core", "core_cand_loads"]
 _STAT_I32 = ["hip_error", "key_bits"]
 
 
 class Stats(C.Structure):
     _fields_ = ([(k, C.c_double) for k in _STAT_F64] + [(k, C.c_int64) for k in _STAT_I64] +
-                [(k, C.c_int32) for k in _STAT_I32])
+                [(k, C.c_int32) for k in _STAT_I32] + [("diag", C.c_int64 * 8)])
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        out = {k: getattr(self, k) for k, _ in self._fields_ if k != "diag"}
+        out["diag"] = list(self.diag)
+        return out
 
 
 class SynthInfo(C.Structure):

@@ -99,8 +99,16 @@ typedef struct dclust_stats {
   int64_t band_flips;      /* FP32 form: rechecks where FP32 alone was wrong */
   int64_t asym_pairs;      /* core-core pairs with P(p,q) != P(q,p)         */
   int64_t pairs_evaluated; /* distance evaluations in the core kernel       */
+  int64_t band_rechecks_core; /* guard-band rechecks inside the core kernel */
+  int64_t core_cand_loads;    /* candidate points loaded by the core kernel */
   int32_t hip_error;       /* last hipError_t, 0 if none                    */
   int32_t key_bits;        /* bits of the sort key                          */
+  /* diagnostics: [0] cells expanded, [1] cells resolved as all-core without
+   * distance tests, [2] expansions, [3] expansions that tested candidate
+   * cells, [4] time of expansion pass 0 (us), [5] sum over expansion
+   * wavefronts of their lifetime (us), [6] longest lifetime (us),
+   * [7] expansion wavefronts */
+  int64_t diag[8];
 } dclust_stats;
 
 typedef struct dclust_handle dclust_handle;
