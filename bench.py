@@ -172,8 +172,8 @@ def main():
     value = n * args.steps / dt / 1e6
     t_core = float(np.mean([s["t_core_ms"] for s in sts]))
     phases = {k: round(float(np.mean([s[k] for s in sts])), 4)
-              for k in ("t_index_ms", "t_core_ms", "t_expand_ms", "t_merge_ms", "t_border_ms",
-                        "t_relabel_ms", "t_total_ms")}
+              for k in ("t_index_ms", "t_core_ms", "t_adj_ms", "t_expand_ms", "t_merge_ms",
+                        "t_border_ms", "t_relabel_ms", "t_total_ms")}
     bpc = 4 if prec == dclust.FP32_GUARD else 8
     bytes_core = b_nbr(st, dim, bpc)
     achieved = bytes_core / (t_core * 1e-3) / 1e9
@@ -194,7 +194,7 @@ def main():
                                         "n_collisions", "n_overflow", "n_passes",
                                         "band_rechecks", "band_flips", "asym_pairs",
                                         "pairs_evaluated", "cand_total", "key_bits",
-                                        "core_cand_loads", "diag")},
+                                        "rec_overflow", "diag")},
     }
     if not args.no_fp64_compare and prec == dclust.FP32_GUARD:
         p64 = dclust.make_params(dim, info["eps"], info["minpts"], r=info["partitions_r"],

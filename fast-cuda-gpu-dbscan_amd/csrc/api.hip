@@ -33,7 +33,7 @@ struct Buf {
   size_t cap = 0;
 };
 
-enum Ev { E_START, E_INDEX, E_CORE0, E_CORE, E_PASS0, E_EXPAND, E_MERGE, E_BORDER, E_END, E_N };
+enum Ev { E_START, E_INDEX, E_CORE0, E_CORE, E_ADJ, E_PASS0, E_EXPAND, E_MERGE, E_BORDER, E_END, E_N };
 
 }  // namespace
 
@@ -80,7 +80,8 @@ enum Slot {
   S_XYZ, S_INST, S_CHUNK, S_BBOX, S_CTL, S_KEY0, S_KEY1, S_VAL0, S_VAL1, S_HIST, S_OFFS,
   S_DTOT, S_PASS, S_CSTART, S_CKEY, S_CINST, S_CM, S_ROWS, S_CELLOF, S_X0, S_X1, S_X2,
   S_G0, S_G1, S_G2, S_ORIG, S_CORE, S_LABEL, S_PARENT, S_COMPMIN, S_PAIRS, S_Q0, S_Q1,
-  S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT, S_PAIRCNT, S_SEGPOS, S_LINES
+  S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT, S_PAIRCNT, S_SEGPOS, S_LINES, S_AMASK,
+  S_HEAVY, S_REC, S_RECN
 };
 
 // line table entries (8 bytes each) above which neighbour lines are found
@@ -133,7 +134,7 @@ RunShape pick_shape(dclust_handle* h, const dclust_params& p, int64_t n, bool f3
   if (p.blocks > 0) {
     r.blocks = p.blocks;
   } else {
-    int per_cu = expand_blocks_per_cu(f32, r.seeds, r.threads);
+    int per_cu = expand_blocks_per_cu(r.seeds, r.threads);
     r.blocks = h->num_cu * std::min(per_cu, 8);
   }
   r.blocks = std::max(8, (r.blocks + 7) / 8 * 8);  // multiple of 8 (XCD remap)
@@ -143,10 +144,11 @@ RunShape pick_shape(dclust_handle* h, const dclust_params& p, int64_t n, bool f3
   r.waves = r.blocks * waves_per_block;
   r.pair_per = p.extra_collision > 0 ? std::max(1, p.extra_collision / waves_per_block) : 512;
   r.pair_cap = (int64_t)r.pair_per * r.waves;
-  // seed batches a wave expands per pass before handing its list on
-  r.budget = 256;
+  // cells a wave expands per pass before handing its list on
+  r.budget = 4096;
   if (const char* e = std::getenv("DCLUST_EXPAND_BUDGET")) r.budget = std::max(1, std::atoi(e));
   (void)n;
+  (void)f32;
   return r;
 }
 
@@ -218,6 +220,13 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
   d.xyz = d_xyz;
   d.pair_cap = shape.pair_cap;
   d.pair_per = shape.pair_per;
+  // neighbour records of non-core points: minpts slots per point (a
+  // non-core point has at most minpts - 1 neighbours outside its own cell,
+  // plus pairs whose box test is asymmetric)
+  int rec_stride = 0;
+  for (int32_t i = 0; i < ninst; ++i) rec_stride = std::max(rec_stride, params[i].minpts);
+  if (rec_stride > kMaxRec) rec_stride = 0;
+  d.rec_stride = rec_stride;
   {
     hipError_t& E = herr;
     d.inst = h->get<Inst>(S_INST, ninst, &E);
@@ -247,6 +256,10 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     d.orig = h->get<int32_t>(S_ORIG, ntot, &E);
     d.core = h->get<uint8_t>(S_CORE, ntot, &E);
     d.cell_label = h->get<int32_t>(S_LABEL, ntot, &E);
+    d.amask = h->get<unsigned long long>(S_AMASK, (size_t)ntot * 2, &E);
+    d.heavy = h->get<int32_t>(S_HEAVY, ntot / (kHeavy + 1) + 1, &E);
+    d.rec_n = h->get<uint8_t>(S_RECN, ntot, &E);
+    if (rec_stride > 0) d.rec = h->get<int32_t>(S_REC, (size_t)ntot * rec_stride, &E);
     d.parent = h->get<int32_t>(S_PARENT, ntot, &E);
     d.compmin = h->get<int32_t>(S_COMPMIN, ntot, &E);
     d.pairs = h->get<int2>(S_PAIRS, shape.pair_cap, &E);
@@ -305,10 +318,13 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     launch_fill_i32(d.compmin, ntot, kNoKey, s);
     launch_fill_u8(d.flag, ntot, 0, s);
     launch_fill_i32(d.pair_cnt, shape.waves, 0, s);
+    launch_fill_i32(d.cell_ncore, ntot, 0, s);
     // the core (neighbour-count) kernel is bracketed by its own events
     HIPCHK(hipEventRecord(h->ev[E_CORE0], s));
-    launch_core(d, f32, grid_cells, s);
+    launch_core(d, f32, s);
     HIPCHK(hipEventRecord(h->ev[E_CORE], s));
+    launch_adj(d, f32, h->num_cu * 16, s);
+    HIPCHK(hipEventRecord(h->ev[E_ADJ], s));
 
     // expansion passes until no seed is pending: pass p drains the queue of
     // pass p-1 (overflowed seed lists, lists left when a wave's pop budget
@@ -316,7 +332,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     launch_seg_init(d, shape.waves, s);
     for (npass = 0; npass < kMaxExpandPasses; ++npass) {
       launch_pass_prep(d, npass, s);
-      launch_expand(d, f32, npass, shape.blocks, shape.seeds, shape.threads, shape.budget, s);
+      launch_expand(d, npass, shape.blocks, shape.seeds, shape.threads, shape.budget, s);
       launch_pass_carry(d, npass, s);
       if (npass == 0) HIPCHK(hipEventRecord(h->ev[E_PASS0], s));
       HIPCHK(hipMemcpyAsync(ctl_h, d.ctl, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
@@ -327,20 +343,21 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     HIPCHK(hipEventRecord(h->ev[E_EXPAND], s));
     launch_merge(d, grid_cells, s);
     HIPCHK(hipEventRecord(h->ev[E_MERGE], s));
-    launch_border(d, f32, grid_cells, ntot, s);
+    launch_border(d, f32, s);
     HIPCHK(hipEventRecord(h->ev[E_BORDER], s));
     launch_relabel(d, ntot, s);
     HIPCHK(hipEventRecord(h->ev[E_END], s));
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ctl_h, d.ctl, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    t_pass0_ms = ev_ms(h, E_CORE, E_PASS0);
+    t_pass0_ms = ev_ms(h, E_ADJ, E_PASS0);
     h->last_n = ntot;
     h->last_core = d.core_out;
     if (stats) {
       stats->t_index_ms = ev_ms(h, E_START, E_CORE0);
       stats->t_core_ms = ev_ms(h, E_CORE0, E_CORE);
-      stats->t_expand_ms = ev_ms(h, E_CORE, E_EXPAND);
+      stats->t_adj_ms = ev_ms(h, E_CORE, E_ADJ);
+      stats->t_expand_ms = ev_ms(h, E_ADJ, E_EXPAND);
       stats->t_merge_ms = ev_ms(h, E_EXPAND, E_MERGE);
       stats->t_border_ms = ev_ms(h, E_MERGE, E_BORDER);
       stats->t_relabel_ms = ev_ms(h, E_BORDER, E_END);
@@ -360,11 +377,11 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
       stats->asym_pairs = ctl_h[C_ASYM];
       stats->pairs_evaluated = ctl_h[C_EVAL];
       stats->band_rechecks_core = ctl_h[C_BAND_CORE];
-      stats->core_cand_loads = ctl_h[C_CANDLOAD];
+      stats->rec_overflow = ctl_h[C_RECOVF];
       stats->diag[0] = ctl_h[C_POPS];
       stats->diag[1] = ctl_h[C_DENSE];
-      stats->diag[2] = ctl_h[C_XCHUNK];
-      stats->diag[3] = ctl_h[C_XREL];
+      stats->diag[2] = ctl_h[C_NHEAVY];
+      stats->diag[3] = ctl_h[C_EDGES];
       stats->diag[4] = (int64_t)(1000.0 * t_pass0_ms);
       stats->diag[5] = ctl_h[C_BUSY_SUM];
       stats->diag[6] = ctl_h[C_BUSY_MAX];

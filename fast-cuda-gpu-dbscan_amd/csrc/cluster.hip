@@ -2,159 +2,108 @@
 // expansion kernel and host merge loop.
 //
 // Grid: cells of side (eps/2)(1+2^-20), so every cell is a clique (all its
-// points are mutual neighbours) and every neighbour lies in the 5^D stencil
-// of cells around a point's cell, stored per cell as 5^(D-1) packed runs of
-// consecutive cells along the last axis ("lines").
+// points are mutual neighbours) and every neighbour of a point lies in the
+// 5^D stencil of cells around its cell.  The stencil is stored per cell as
+// 5^(D-1) packed runs ("lines") of at most 5 consecutive non-empty cells
+// along the last axis; line r, cell k of a cell's stencil is its stencil
+// slot 5r+k.
 //
-//   k_core     neighbour counts -> core flags.  A cell holding >= minpts
-//              points is all core without a distance test; otherwise the
-//              count starts at the cell's own size and streams the other
-//              stencil cells with early exit at minpts.  (dbscan.cu:331-364
-//              counts neighbours while expanding; here the count is a
-//              separate pass so every later kernel knows which candidates
-//              are core.)
-//   k_expand   persistent seed-list expansion (dbscan.cu:254-387) over
-//              cells: each wavefront owns a chain and a seed list of cells in
-//              LDS, pops a cell, looks up the labels of its stencil cells,
-//              tests only cells that hold core points and are not yet its
-//              own, claims adjacent unclaimed cells with one CAS each
-//              (MarkAsCandidate, dbscan.cu:189-222), pushes them with
-//              ballot/mbcnt and records collisions with other chains as
-//              pairs in its own buffer (extraCollision, dbscan.cu:202-211).
-//              Seed-list overflow and lists left when a wave's pop budget
-//              runs out go to a correction queue that the next pass expands,
-//              instead of being dropped (dbscan.cu:196-199).
+//   k_core       one lane per point: neighbour count with early exit at
+//                minpts -> core flag.  (dbscan.cu:331-369 counts while it
+//                expands; here the count is its own pass, so every later
+//                kernel knows which points are core.)  A cell holding
+//                >= minpts points is all core without a distance test.  A
+//                point that stays below minpts has seen all its neighbours
+//                and records them for the border pass.
+//   k_adj        cell adjacency: bit 5r+k of a core cell's 128-bit mask is
+//                set when stencil slot 5r+k holds a core point within eps of
+//                one of the cell's core points.  Light cells: one lane per
+//                core point.  Heavy cells (> kHeavy points): one wavefront
+//                per cell (k_adj_heavy), which also sets the mirrored bit of
+//                its light neighbours.
+//   k_expand     persistent seed-list expansion (dbscan.cu:254-387) over
+//                cells: each wavefront owns a seed list of cells in LDS and
+//                pops up to 64 of them at once, one per lane; each lane
+//                claims the adjacent unclaimed cells of its cell with one CAS
+//                each (MarkAsCandidate, dbscan.cu:189-222) and pushes them;
+//                cells already claimed by another chain become collision
+//                pairs in the wave's own buffer (extraCollision,
+//                dbscan.cu:202-211).  Seed-list overflow and lists left when
+//                a wave's budget runs out go to a correction queue that the
+//                next pass expands, instead of being dropped
+//                (dbscan.cu:196-199).
 //   k_merge_lds / k_union_pairs / k_flatten   GPU atomic union-find over
-//              chain ids (MonitorSeedPoints' CPU BFS over the collision
-//              matrix and thrust::replace relabels, dbscan.cu:60-125).
+//                chain ids (MonitorSeedPoints' CPU BFS over the collision
+//                matrix and thrust::replace relabels, dbscan.cu:60-125).
 //   k_compmin / k_keys / k_border / rank scan / k_output   canonical labels:
-//              cluster id = rank of the component's minimum core input
-//              index; border = minimum id over adjacent cores; noise = 0
-//              (GetDbscanResult, dbscan.cu:225-251, writes sorted labels
-//              instead; this writes per-point labels).
+//                cluster id = rank of the component's minimum core input
+//                index; border = minimum id over adjacent cores; noise = 0
+//                (GetDbscanResult, dbscan.cu:225-251, writes sorted labels
+//                instead; this writes per-point labels).
 //
 // Neighbour predicate P(p,q), FP64 form (exactly the reference oracle's,
 // cpu-dbscan/dbscan_rtree_exp.cpp:351-358,420-438 and Rtree.h:1573-1587):
 // closed box fl(p-eps) <= q <= fl(p+eps) on every axis, and
 // fl(fl(fl(dx0^2) + fl(dx1^2)) [+ fl(dx2^2)]) <= fl(eps*eps).  This file is
-// compiled with -ffp-contract=off so no FMA is formed.
+// compiled with -ffp-contract=off so no FMA is formed.  d^2 is symmetric in
+// p and q (fl(p-q) = -fl(q-p)); only the box test can differ between the two
+// directions.  Two core points are joined when d^2 <= eps^2 and the box test
+// holds from at least one side (the oracle's union runs from every core
+// point, cpu-dbscan/dbscan_rtree_exp.cpp:379-404).
+#include <climits>
+
 #include "internal.h"
 #include "kernels.h"
 
 namespace dclust {
 
-constexpr int kSegMax = 128;  // segments of one stream (per-wave LDS table)
+constexpr int kQCache = 256;  // query points a heavy-cell wavefront keeps in LDS
 
-// ------------------------------------------------------- segment streams --
-// A stream is the concatenation of up to kSegMax contiguous index ranges
-// (runs of points, or runs of cells).  pref[k] = stream position of the
-// first element of segment k; off[k] = segment start - pref[k].
-struct Seg {
-  int32_t* pref;
-  int32_t* off;
-  int K;
-  int32_t M;
+template <bool F32>
+struct Batch {
+  static constexpr int value = F32 ? 8 : 4;  // candidates loaded per lane step
 };
 
-// Each lane contributes one range (start, len); empty ranges are dropped
-// and the order of lanes is kept.
-__device__ __forceinline__ void seg_build(int32_t start, int32_t len, int32_t* pref,
-                                          int32_t* off, Seg& s) {
-  const bool has = len > 0;
-  const unsigned long long m = ballot(has);
-  const int32_t incl = wave_incl_scan(len);
-  const int32_t excl = incl - len;
-  if (has) {
-    const int k = mbcnt(m);
-    pref[k] = excl;
-    off[k] = start - excl;
-  }
-  s.pref = pref;
-  s.off = off;
-  s.K = __popcll(m);
-  s.M = rl_i(incl, 63);
-  wave_sync();
-}
-
-// element at stream position m (m < M); k receives its segment index
-__device__ __forceinline__ int32_t seg_at(const Seg& s, int32_t m, int& k) {
-  int lo = 0, hi = s.K - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (s.pref[mid] <= m) lo = mid; else hi = mid - 1;
-  }
-  k = lo;
-  return s.off[lo] + m;
-}
-
-// The stencil of cell c as runs of cells: lane r < NR holds line r, lane NR
-// the part of the centre line after c.  With skip_self the cell itself is
-// left out (the centre line is split around it).
-template <int NR>
-__device__ __forceinline__ void stencil_cells(const Dev& d, int32_t c, bool skip_self,
-                                              int32_t& st, int32_t& len) {
-  const int lane = lane_id();
-  st = 0;
-  len = 0;
-  if (lane < NR) {
-    const int32_t pk = d.rows[(int64_t)c * kMaxRows + lane];
-    st = pk & kRowCellMask;
-    len = pk >> kRowCntShift;
-  }
-  if (skip_self) {
-    const int32_t s0 = rl_i(st, 0), e0 = s0 + rl_i(len, 0);  // centre line, contains c
-    if (lane == 0) len = c - s0;
-    if (lane == NR) {
-      st = c + 1;
-      len = e0 - (c + 1);
-    }
-  }
-}
-
-// The stencil of cell c as runs of points.
-template <int NR>
-__device__ __forceinline__ void stencil_points(const Dev& d, int32_t c, bool skip_self,
-                                               int32_t* pref, int32_t* off, Seg& s) {
-  int32_t st, len;
-  stencil_cells<NR>(d, c, skip_self, st, len);
-  int32_t ps = 0, pl = 0;
-  if (len > 0) {
-    ps = d.cell_start[st];
-    pl = d.cell_start[st + len] - ps;
-  }
-  seg_build(ps, pl, pref, off, s);
-}
+struct Tally {
+  long long band = 0, flip = 0, asym = 0, eval = 0;
+};
 
 // ------------------------------------------------------------ predicates --
-// exact FP64 predicate centred on p (box bounds lo/hi precomputed from p)
 template <int D>
-__device__ __forceinline__ bool pred64(const double* p, const double* lo, const double* hi,
-                                       const double* q, double eps2) {
-  bool in = true;
+__device__ __forceinline__ void load_x(const Dev& d, int32_t i, double* x) {
 #pragma unroll
-  for (int k = 0; k < D; ++k) in = in && !(lo[k] > q[k] || q[k] > hi[k]);
-  double t = p[0] - q[0];
+  for (int k = 0; k < D; ++k) x[k] = d.X[k][i];
+}
+template <int D>
+__device__ __forceinline__ void load_g(const Dev& d, int32_t i, uint32_t* g) {
+#pragma unroll
+  for (int k = 0; k < D; ++k) g[k] = d.G[k][i];
+}
+
+// fl(fl(fl(d0^2) + fl(d1^2)) [+ fl(d2^2)]), left to right
+template <int D>
+__device__ __forceinline__ double d2_64(const double* a, const double* b) {
+  double t = a[0] - b[0];
   double s = t * t;
 #pragma unroll
   for (int k = 1; k < D; ++k) {
-    double u = p[k] - q[k];
+    double u = a[k] - b[k];
     double v = u * u;
     s = s + v;
   }
-  return in && s <= eps2;
+  return s;
 }
 
-// exact predicate centred on p: box bounds fl(p -+ eps) formed here
+// the oracle's box of a range query centred on c: fl(c-eps) <= q <= fl(c+eps)
 template <int D>
-__device__ __forceinline__ bool pred64c(const double* p, const double* q, double eps,
-                                        double eps2) {
-  double lo[D], hi[D];
+__device__ __forceinline__ bool box_64(const double* c, const double* q, double eps) {
+  bool in = true;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    lo[k] = p[k] - eps;
-    hi[k] = p[k] + eps;
+    const double lo = c[k] - eps, hi = c[k] + eps;
+    in = in && lo <= q[k] && q[k] <= hi;
   }
-  return pred64<D>(p, lo, hi, q, eps2);
+  return in;
 }
 
 // FP32 squared distance from fixed-point coordinates (units of w*2^-29).
@@ -172,233 +121,9 @@ __device__ __forceinline__ float d2fix(const uint32_t* a, const uint32_t* b) {
   return s;
 }
 
-struct Tally {
-  long long band = 0, flip = 0, asym = 0, eval = 0, loads = 0;
-  long long pops = 0, batch = 0, xchunk = 0, xrel = 0;
-};
-
-// Query data held by a lane (one query per lane); the iteration over
-// queries reads one lane's copy into wave-uniform registers.  The FP64 form
-// keeps the FP64 coordinates, the FP32 form only the fixed-point ones (its
-// rare guard-band rechecks load the FP64 coordinates from memory).
-template <int D, bool F32>
-struct Query {
-  double x[D];
-  uint32_t g[D];
-};
-
-template <int D, bool F32>
-__device__ __forceinline__ void load_query_lane(const Dev& d, int32_t pt, bool active,
-                                                Query<D, F32>& q) {
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    if (F32)
-      q.g[k] = active ? d.G[k][pt] : 0u;
-    else
-      q.x[k] = active ? d.X[k][pt] : 0.0;
-  }
-}
-
-template <int D, bool F32>
-__device__ __forceinline__ void read_query(const Query<D, F32>& mine, int qi, Query<D, F32>& u) {
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    if (F32)
-      u.g[k] = rl_u(mine.g[k], qi);
-    else
-      u.x[k] = rl_d(mine.x[k], qi);
-  }
-}
-
-// Candidate coordinates held by a lane.
-template <int D, bool F32>
-struct Cand {
-  double x[D];
-  uint32_t g[D];
-};
-
-template <int D, bool F32>
-__device__ __forceinline__ void load_cand(const Dev& d, int32_t pt, bool active, Cand<D, F32>& c) {
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    if (F32)
-      c.g[k] = active ? d.G[k][pt] : 0u;
-    else
-      c.x[k] = active ? d.X[k][pt] : 0.0;
-  }
-}
-
-// FP32 decision for the fixed-point pair (a, b), symmetric in a and b.
-// Returns true inside; sets amb for the guard band.
-template <int D>
-__device__ __forceinline__ bool f32_decide(const Inst& I, const uint32_t* a, const uint32_t* b,
-                                           bool active, bool& amb, float& s) {
-  s = d2fix<D>(a, b);
-  const bool in = s < I.d2_in;
-  amb = active && !in && s <= I.d2_out;
-  return in;
-}
-
-// P(query -> candidate): the query (point index qpt, wave-uniform) is the
-// centre, as in its own neighbour count or expansion.  The FP32 form
-// decides outside the guard band and re-decides inside it with the exact
-// FP64 predicate (FP64 coordinates fetched only then).  With ASYM, pairs
-// whose box test differs between the two directions are counted (d^2 is
-// symmetric; outside the guard band the box test cannot differ).
-template <int D, bool F32, bool ASYM>
-__device__ __forceinline__ bool pred_qc(const Dev& d, const Inst& I, int32_t qpt,
-                                        const Query<D, F32>& uq, const Cand<D, F32>& c,
-                                        int32_t pt, bool active, Tally& t) {
-  if (!F32) {
-    const bool fwd = active && pred64c<D>(uq.x, c.x, I.eps, I.eps2);
-    if (ASYM) {
-      const bool rev = active && pred64c<D>(c.x, uq.x, I.eps, I.eps2);
-      t.asym += fwd != rev;
-    }
-    return fwd;
-  }
-  bool amb;
-  float s;
-  bool in = f32_decide<D>(I, uq.g, c.g, active, amb, s);
-  if (ballot(amb)) {
-    double px[D], qx[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) px[k] = d.X[k][qpt];
-    if (amb) {
-#pragma unroll
-      for (int k = 0; k < D; ++k) qx[k] = d.X[k][pt];
-      bool ex = pred64c<D>(px, qx, I.eps, I.eps2);
-      if (ASYM) t.asym += ex != pred64c<D>(qx, px, I.eps, I.eps2);
-      t.band++;
-      t.flip += (ex != (s <= I.d2_mid));
-      in = ex;
-    }
-  }
-  return active && in;
-}
-
-// ------------------------------------------------------------------ core --
-template <int D, bool F32>
-__device__ void core_cell(const Dev& d, const Inst& I, int32_t c, int32_t* pref, int32_t* off,
-                          Tally& t, long long& ncore, long long& dense) {
-  constexpr int NR = D == 3 ? 25 : 5;
-  const int lane = lane_id();
-  const int32_t qs = d.cell_start[c], qe = d.cell_start[c + 1];
-  const int32_t nc = qe - qs;
-  if (nc >= I.minpts) {
-    // a clique of at least minpts points: every point is core
-    for (int32_t q = qs + lane; q < qe; q += 64) d.core[q] = 1;
-    if (lane == 0) d.cell_ncore[c] = nc;
-    ncore += nc;
-    dense++;
-    return;
-  }
-  Seg st;
-  stencil_points<NR>(d, c, true, pref, off, st);
-  int32_t ncc = 0;
-  for (int32_t q0 = qs; q0 < qe; q0 += 64) {
-    const int nq = (qe - q0) < 64 ? (qe - q0) : 64;
-    Query<D, F32> mq;
-    load_query_lane<D, F32>(d, q0 + lane, lane < nq, mq);
-    int cnt = nc;  // the own cell is a clique, the point itself included
-    unsigned long long done = nq == 64 ? 0ull : (~0ull << nq);
-    for (int32_t m0 = 0; m0 < st.M && done != ~0ull; m0 += 64) {
-      const int32_t m = m0 + lane;
-      const bool valid = m < st.M;
-      int k;
-      const int32_t pt = valid ? seg_at(st, m, k) : 0;
-      Cand<D, F32> cd;
-      load_cand<D, F32>(d, pt, valid, cd);
-      unsigned long long todo = ~done;
-      const int nvalid = __popcll(ballot(valid));
-      t.eval += (long long)__popcll(todo) * nvalid;
-      t.loads += nvalid;
-      while (todo) {
-        const int qi = __ffsll((long long)todo) - 1;
-        todo &= todo - 1;
-        Query<D, F32> uq;
-        read_query<D, F32>(mq, qi, uq);
-        bool hit = pred_qc<D, F32, false>(d, I, q0 + qi, uq, cd, pt, valid, t);
-        int pc = __popcll(ballot(hit));
-        cnt += (lane == qi) ? pc : 0;
-      }
-      done |= ballot(cnt >= I.minpts);
-    }
-    const bool isc = lane < nq && cnt >= I.minpts;
-    if (lane < nq) d.core[q0 + lane] = isc;
-    ncc += __popcll(ballot(isc));
-  }
-  if (lane == 0) d.cell_ncore[c] = ncc;
-  ncore += ncc;
-}
-
-template <bool F32>
-__global__ __launch_bounds__(256) void k_core(Dev d) {
-  __shared__ int32_t s_seg[4][2][kSegMax];
-  const int wv = threadIdx.x >> 6;
-  const int32_t C = (int32_t)d.ctl[C_NCELL];
-  const int nwb = blockDim.x >> 6;
-  const int64_t gw = (int64_t)xcd_block(blockIdx.x, gridDim.x) * nwb + wv;
-  const int64_t W = (int64_t)gridDim.x * nwb;
-  constexpr int CH = 16;  // cells per wave step
-  Tally t;
-  long long ncore = 0, dense = 0;
-  for (int64_t c0 = gw * CH; c0 < C; c0 += W * CH) {
-    const int32_t ce = (int32_t)((c0 + CH) < C ? (c0 + CH) : C);
-    for (int32_t c = (int32_t)c0; c < ce; ++c) {
-      const Inst& I = d.inst[d.cell_inst[c]];
-      if (I.dim == 3)
-        core_cell<3, F32>(d, I, c, s_seg[wv][0], s_seg[wv][1], t, ncore, dense);
-      else
-        core_cell<2, F32>(d, I, c, s_seg[wv][0], s_seg[wv][1], t, ncore, dense);
-    }
-  }
-  if (lane_id() == 0) {
-    if (ncore) atomicAdd((unsigned long long*)&d.ctl[C_NCORE], (unsigned long long)ncore);
-    if (dense) atomicAdd((unsigned long long*)&d.ctl[C_DENSE], (unsigned long long)dense);
-    if (t.eval) atomicAdd((unsigned long long*)&d.ctl[C_EVAL], (unsigned long long)t.eval);
-    if (t.loads) atomicAdd((unsigned long long*)&d.ctl[C_CANDLOAD], (unsigned long long)t.loads);
-  }
-  wave_add(&d.ctl[C_BAND_CORE], t.band);
-  wave_add(&d.ctl[C_BAND], t.band);
-  wave_add(&d.ctl[C_FLIP], t.flip);
-}
-
-// ---------------------------------------------------------------- expand --
-// Per-wave LDS working set of the expansion.
-struct ExpLds {
-  int32_t* pref;   // [kSegMax] stream table
-  int32_t* off;    // [kSegMax]
-  int32_t* rcell;  // [kSegMax] cells to test (core points, not own chain)
-  int32_t* rlab;   // [kSegMax] their labels when looked up
-  int32_t* radj;   // [kSegMax] 1 once found adjacent to the expanded cell
-  int32_t* qlist;  // [64] query points (core points of the expanded cell)
-};
-
-struct SeedList {
-  int32_t* list;  // seed cells
-  int S;
-  int sl;
-  int32_t* qout;
-  long long* qout_n;
-  int32_t cache;  // lanes 0..7: recently recorded foreign chains
-  int cache_pos;
-  int2* pbuf;     // this wave's collision-pair buffer (EXTRA_COLLISION_SIZE
-  int pcnt;       // per chain in dbscan.cu:202-211), no shared counter
-  int pcap;
-};
-
-// Record a chain-chain collision in the wave's own buffer; when it is full
-// the two chains are merged inline (same final result).  Called by lane 0.
-__device__ __forceinline__ void record_pair(const Dev& d, SeedList& L, int32_t a, int32_t b) {
-  if (L.pcnt < L.pcap)
-    L.pbuf[L.pcnt] = make_int2(a, b);
-  else
-    uf_union(d.parent, a, b);
-}
-
-// squared FP32 distance (fixed point) from point g to the box [lo, hi]
-// (box of one cell's points: no wrap inside a cell)
+// FP32 squared distance (fixed point) from point g to the box [lo, hi] of
+// points of one cell (no wrap inside a cell).  Never above d2fix(g, q) for
+// any q in the box, so d2box > d2_out rules out every pair.
 template <int D>
 __device__ __forceinline__ float d2box(const uint32_t* g, const uint32_t* lo, const uint32_t* hi) {
   float s = 0.f;
@@ -413,262 +138,595 @@ __device__ __forceinline__ float d2box(const uint32_t* g, const uint32_t* lo, co
   return s;
 }
 
-// Expand cell A of chain `own`: find the stencil cells with core points
-// that are not yet own, decide which are adjacent (some core pair within
-// eps), claim the unclaimed ones and record the foreign ones.
+// Scalar parameters of one instance, read once per lane.
+struct Par {
+  double eps, eps2;
+  float d2_in, d2_out, d2_mid;
+  int minpts;
+  __device__ __forceinline__ explicit Par(const Inst& I)
+      : eps(I.eps), eps2(I.eps2), d2_in(I.d2_in), d2_out(I.d2_out), d2_mid(I.d2_mid),
+        minpts(I.minpts) {}
+};
+
+// Core-core relation of the pair (a, b): d^2 <= eps^2 and the box test from
+// at least one side.  FP32 form: decided by the fixed-point distance outside
+// the guard band, exactly in FP64 inside it.
 template <int D, bool F32>
-__device__ void expand_cell(const Dev& d, const Inst& I, int32_t A, int32_t own, SeedList& L,
-                            ExpLds& X, Tally& t, long long& ovf) {
-  constexpr int NR = D == 3 ? 25 : 5;
-  const int lane = lane_id();
-  // 1. stencil cells (A excluded) and the ones worth testing
-  Seg cs;
-  {
-    int32_t st, len;
-    stencil_cells<NR>(d, A, true, st, len);
-    seg_build(st, len, X.pref, X.off, cs);
-  }
-  int nrel = 0;
-  int32_t rcell[2] = {0, 0}, rlab[2] = {0, 0};
-  bool rel[2] = {false, false};
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int32_t j = h * 64 + lane;
-    if (j < cs.M) {
-      int k;
-      const int32_t cj = seg_at(cs, j, k);
-      const int32_t lab = d.cell_label[cj];
-      const int32_t nco = d.cell_ncore[cj];
-      rel[h] = nco > 0 && lab != own;
-      rcell[h] = cj;
-      rlab[h] = lab;
-    }
-  }
-  wave_sync();  // the cell table is about to be overwritten
-  t.xchunk++;
-  {
-    const unsigned long long m0 = ballot(rel[0]), m1 = ballot(rel[1]);
-    const int n0 = __popcll(m0);
-    nrel = n0 + __popcll(m1);
-    if (nrel == 0) return;
-    t.xrel++;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      if (rel[h]) {
-        const int k = (h ? n0 + mbcnt(m1) : mbcnt(m0));
-        X.rcell[k] = rcell[h];
-        X.rlab[k] = rlab[h];
-        X.radj[k] = 0;
-      }
-    }
-    wave_sync();
-  }
-  // point runs of the cells to test, one segment per cell (cell k <-> segment k)
-  Seg ps;
-  {
-    int32_t s0 = 0, l0 = 0, s1 = 0, l1 = 0;
-    if (lane < nrel) {
-      const int32_t cc = X.rcell[lane];
-      s0 = d.cell_start[cc];
-      l0 = d.cell_start[cc + 1] - s0;
-    }
-    if (lane + 64 < nrel) {
-      const int32_t cc = X.rcell[lane + 64];
-      s1 = d.cell_start[cc];
-      l1 = d.cell_start[cc + 1] - s1;
-    }
-    // two rounds of seg_build into one table
-    const unsigned long long m0 = ballot(l0 > 0);
-    const int32_t i0 = wave_incl_scan(l0);
-    const int32_t e0 = i0 - l0;
-    if (l0 > 0) {
-      const int k = mbcnt(m0);
-      X.pref[k] = e0;
-      X.off[k] = s0 - e0;
-    }
-    const int K0 = __popcll(m0);
-    const int32_t M0 = rl_i(i0, 63);
-    const unsigned long long m1 = ballot(l1 > 0);
-    const int32_t i1 = wave_incl_scan(l1);
-    const int32_t e1 = M0 + i1 - l1;
-    if (l1 > 0) {
-      const int k = K0 + mbcnt(m1);
-      X.pref[k] = e1;
-      X.off[k] = s1 - e1;
-    }
-    ps.pref = X.pref;
-    ps.off = X.off;
-    ps.K = K0 + __popcll(m1);
-    ps.M = M0 + rl_i(i1, 63);
-    wave_sync();
-  }
-  // 2. queries: core points of A, up to 64 at a time
-  const int32_t as = d.cell_start[A], ae = d.cell_start[A + 1];
-  const bool a_all_core = d.cell_ncore[A] == ae - as;
-  for (int32_t q0 = as; q0 < ae; q0 += 64) {
-    const bool qa = q0 + lane < ae && (a_all_core || d.core[q0 + lane]);
-    const unsigned long long qm = ballot(qa);
-    if (!qm) continue;
-    if (qa) X.qlist[mbcnt(qm)] = q0 + lane;
-    wave_sync();
-    const int nq = __popcll(qm);
-    const int32_t qpt = lane < nq ? X.qlist[lane] : 0;
-    Query<D, F32> mq;
-    load_query_lane<D, F32>(d, qpt, lane < nq, mq);
-    // bounding box of the queries (fixed point; one cell, so no wrap)
-    uint32_t blo[D], bhi[D];
+__device__ __forceinline__ bool edge_pair(const Dev& d, const Par& P, int32_t a, int32_t b,
+                                          const uint32_t* ag, const uint32_t* bg,
+                                          const double* ax, const double* bx, Tally& t) {
+  double xa[D], xb[D];
+  if (F32) {
+    const float s = d2fix<D>(ag, bg);
+    if (s < P.d2_in) return true;
+    if (s > P.d2_out) return false;
+    t.band++;
+    load_x<D>(d, a, xa);
+    load_x<D>(d, b, xb);
+  } else {
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-      const uint32_t g = lane < nq ? d.G[k][qpt] : 0u;
-      blo[k] = wave_min(lane < nq ? g : 0xffffffffu);
-      bhi[k] = wave_max(lane < nq ? g : 0u);
-    }
-    // 3. stream the candidate cells' points
-    for (int32_t m0 = 0; m0 < ps.M; m0 += 64) {
-      const int32_t m = m0 + lane;
-      const bool valid = m < ps.M;
-      int k = 0;
-      const int32_t pt = valid ? seg_at(ps, m, k) : 0;
-      bool act = valid && !X.radj[k];
-      if (!ballot(act)) continue;
-      uint32_t cg[D];
-#pragma unroll
-      for (int a = 0; a < D; ++a) cg[a] = act ? d.G[a][pt] : 0u;
-      if (act) act = d.core[pt] && !(d2box<D>(cg, blo, bhi) > I.d2_out);
-      if (!ballot(act)) continue;
-      Cand<D, F32> cd;
-      if (F32) {
-#pragma unroll
-        for (int a = 0; a < D; ++a) cd.g[a] = cg[a];
-      } else {
-        load_cand<D, F32>(d, pt, act, cd);
-      }
-      bool hit = false;
-      for (int qi = 0; qi < nq; ++qi) {
-        Query<D, F32> uq;
-        read_query<D, F32>(mq, qi, uq);
-        hit |= pred_qc<D, F32, true>(d, I, rl_i(qpt, qi), uq, cd, pt, act && !hit, t);
-        if (!ballot(act && !hit)) break;
-      }
-      if (hit) X.radj[k] = 1;
-      wave_sync();
+      xa[k] = ax[k];
+      xb[k] = bx[k];
     }
   }
-  // 4. claims and collisions
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int j = h * 64 + lane;
-    const bool adj = j < nrel && X.radj[j];
-    const int32_t cj = adj ? X.rcell[j] : 0;
-    int32_t lab = adj ? X.rlab[j] : own;
-    bool claimed = false;
-    if (adj && lab == kUnclaimed) {
-      const int32_t old = atomicCAS(&d.cell_label[cj], kUnclaimed, own);
-      claimed = old == kUnclaimed;
-      lab = claimed ? own : old;
-    }
-    bool coll = adj && !claimed && lab != own;
-    unsigned long long cm = ballot(coll);
-    while (cm) {
-      const int l = __ffsll((long long)cm) - 1;
-      const int32_t v = rl_i(lab, l);
-      const bool seen = ballot(lane < 8 && L.cache == v) != 0;
-      if (!seen) {
-        if (lane == 0) record_pair(d, L, own, v);
-        L.pcnt++;
-        if (lane == L.cache_pos) L.cache = v;
-        L.cache_pos = (L.cache_pos + 1) & 7;
-      }
-      coll = coll && lab != v;
-      cm = ballot(coll);
-    }
-    const unsigned long long pm = ballot(claimed);
-    if (pm) {
-      const int cnt = __popcll(pm);
-      const int pos = L.sl + mbcnt(pm);
-      const int over = L.sl + cnt - L.S;
-      long long base = 0;
-      if (over > 0) {
-        if (lane == 0) base = (long long)atomicAdd((unsigned long long*)L.qout_n, (unsigned long long)over);
-        base = __shfl(base, 0, 64);
-        ovf += over;
-      }
-      if (claimed) {
-        if (pos < L.S)
-          L.list[pos] = cj;
-        else
-          L.qout[base + pos - L.S] = cj;
-      }
-      L.sl = (L.sl + cnt) < L.S ? (L.sl + cnt) : L.S;
-    }
-  }
-  wave_sync();
+  const double dd = d2_64<D>(xa, xb);
+  if (!(dd <= P.eps2)) return false;
+  const bool f = box_64<D>(xa, xb, P.eps), r = box_64<D>(xb, xa, P.eps);
+  t.asym += (f != r) && a < b;
+  return f || r;
 }
 
+// ------------------------------------------------------ candidate streams --
+// Up to five index ranges [a_j, a_j + l_j) read as one stream: position t
+// lies in range j = #{k >= 1 : P_k <= t}, P_k = l_0 + ... + l_{k-1}.
+struct Stream5 {
+  int32_t a[5];
+  int32_t P[5];
+  int32_t tot;
+  __device__ __forceinline__ void set(const int32_t* aa, const int32_t* ll) {
+    int32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      a[j] = aa[j];
+      P[j] = s;
+      s += ll[j];
+    }
+    tot = s;
+  }
+  __device__ __forceinline__ int32_t at(int32_t t, int& j) const {
+    int32_t q = a[0] + t;
+    j = 0;
+#pragma unroll
+    for (int k = 1; k < 5; ++k)
+      if (t >= P[k]) {
+        q = a[k] + (t - P[k]);
+        j = k;
+      }
+    return q;
+  }
+};
+
+// point ranges of five stencil lines
+__device__ __forceinline__ void line_group(const Dev& d, const int32_t* rows5, int32_t* a,
+                                           int32_t* l) {
+  int32_t pk[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) pk[j] = rows5[j];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const int32_t len = pk[j] >> kRowCntShift, st = pk[j] & kRowCellMask;
+    int32_t x = 0, y = 0;
+    if (len) {
+      x = d.cell_start[st];
+      y = d.cell_start[st + len];
+    }
+    a[j] = x;
+    l[j] = y - x;
+  }
+}
+
+__device__ __forceinline__ bool mask_has(unsigned long long m0, unsigned long long m1, int b) {
+  return ((b < 64 ? (m0 >> b) : (m1 >> (b - 64))) & 1ull) != 0;
+}
+__device__ __forceinline__ void mask_set(unsigned long long& m0, unsigned long long& m1, int b) {
+  if (b < 64)
+    m0 |= 1ull << b;
+  else
+    m1 |= 1ull << (b - 64);
+}
+
+// Per-lane dispatch on the instance dimension (a wavefront can straddle two
+// instances of a batch).
+#define DCLUST_BY_DIM(act, dim, CALL3, CALL2)      \
+  do {                                             \
+    if (ballot((act) && (dim) == 3)) {             \
+      if ((act) && (dim) == 3) { CALL3; }          \
+    }                                              \
+    if (ballot((act) && (dim) == 2)) {             \
+      if ((act) && (dim) == 2) { CALL2; }          \
+    }                                              \
+  } while (0)
+
+// ------------------------------------------------------------------ core --
+// Neighbour count of point p with early exit at minpts.  Every candidate
+// pair with d^2 <= eps^2 (either box direction) is recorded while the count
+// is below minpts; a point that ends non-core thereby holds all its
+// neighbours outside its own cell (at most minpts - 1 plus box-asymmetric
+// pairs).
+template <int D, bool F32>
+__device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int32_t qs,
+                          int32_t qe, int R, Tally& t, bool& isc) {
+  constexpr int NR = D == 3 ? 25 : 5;
+  constexpr int B = Batch<F32>::value;
+  int cnt = qe - qs;  // the own cell is a clique, p itself included
+  if (cnt >= P.minpts) {
+    isc = true;
+    return;
+  }
+  uint32_t pg[D];
+  double px[D], lo[D], hi[D];
+  if (F32) {
+    load_g<D>(d, p, pg);
+  } else {
+    load_x<D>(d, p, px);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      lo[k] = px[k] - P.eps;
+      hi[k] = px[k] + P.eps;
+    }
+  }
+  int32_t* rec = d.rec + (int64_t)p * R;
+  int nrec = 0;
+  long long ev = 0;
+  const int32_t* rows = d.rows + (int64_t)c * kMaxRows;
+  for (int g = 0; g < NR && cnt < P.minpts; g += 5) {
+    int32_t aa[5], ll[5];
+    line_group(d, rows + g, aa, ll);
+    Stream5 s;
+    s.set(aa, ll);
+    for (int32_t t0 = 0; t0 < s.tot && cnt < P.minpts; t0 += B) {
+      int32_t qi[B];
+      uint32_t cg[B][D];
+      double cx[B][D];
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        int jr;
+        qi[j] = t0 + j < s.tot ? s.at(t0 + j, jr) : p;
+        if (F32)
+          load_g<D>(d, qi[j], cg[j]);
+        else
+          load_x<D>(d, qi[j], cx[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        const int32_t q = qi[j];
+        if (t0 + j < s.tot && (q < qs || q >= qe)) {
+          bool fwd, sym;
+          if (F32) {
+            const float sd = d2fix<D>(pg, cg[j]);
+            if (sd < P.d2_in) {
+              fwd = sym = true;
+            } else if (sd > P.d2_out) {
+              fwd = sym = false;
+            } else {
+              double ax[D], bx[D];
+              load_x<D>(d, p, ax);
+              load_x<D>(d, q, bx);
+              const double dd = d2_64<D>(ax, bx);
+              sym = dd <= P.eps2;
+              fwd = sym && box_64<D>(ax, bx, P.eps);
+              t.band++;
+              t.flip += fwd != (sd <= P.d2_mid);
+            }
+          } else {
+            const double dd = d2_64<D>(px, cx[j]);
+            sym = dd <= P.eps2;
+            bool in = true;
+#pragma unroll
+            for (int k = 0; k < D; ++k) in = in && lo[k] <= cx[j][k] && cx[j][k] <= hi[k];
+            fwd = sym && in;
+          }
+          cnt += fwd;
+          if (sym) {
+            if (nrec < R) rec[nrec] = q;
+            ++nrec;
+          }
+          ++ev;
+        }
+      }
+    }
+  }
+  isc = cnt >= P.minpts;
+  if (!isc && R > 0) d.rec_n[p] = (uint8_t)(nrec < 255 ? nrec : 255);
+  t.eval += ev;
+}
+
+template <bool F32>
+__global__ __launch_bounds__(256) void k_core(Dev d) {
+  const int64_t pl = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const bool act = pl < d.n;
+  const int32_t p = act ? (int32_t)pl : 0;
+  const int lane = lane_id();
+  int32_t c = 0, qs = 0, qe = 0;
+  int dim = 0;
+  const Inst* I = d.inst;
+  if (act) {
+    c = d.cell_of[p];
+    qs = d.cell_start[c];
+    qe = d.cell_start[c + 1];
+    I = d.inst + d.cell_inst[c];
+    dim = I->dim;
+  }
+  Tally t;
+  bool isc = false;
+  const int R = d.rec_stride;
+  const Par P(*I);
+  DCLUST_BY_DIM(act, dim, (core_lane<3, F32>(d, P, p, c, qs, qe, R, t, isc)),
+                (core_lane<2, F32>(d, P, p, c, qs, qe, R, t, isc)));
+  if (act) d.core[p] = isc;
+  // per-cell core counts: the lanes of one cell form a contiguous run
+  const bool head = act && (p == qs || lane == 0);
+  const unsigned long long mh = ballot(head), mc = ballot(isc);
+  long long dense = 0;
+  if (head) {
+    const unsigned long long above = lane == 63 ? 0ull : (mh & (~0ull << (lane + 1)));
+    const int nxt = above ? __ffsll((long long)above) - 1 : 64;
+    const unsigned long long seg = (nxt == 64 ? ~0ull : ((1ull << nxt) - 1ull)) & (~0ull << lane);
+    const int k = __popcll(mc & seg);
+    if (k) atomicAdd(&d.cell_ncore[c], k);
+    if (p == qs) {
+      // first point of its cell: clear the adjacency mask, list heavy cells
+      d.amask[2 * (int64_t)c] = 0ull;
+      d.amask[2 * (int64_t)c + 1] = 0ull;
+      dense = (qe - qs) >= P.minpts;
+      if (qe - qs > kHeavy) {
+        const long long h = (long long)atomicAdd((unsigned long long*)&d.ctl[C_NHEAVY], 1ull);
+        d.heavy[h] = c;
+      }
+    }
+  }
+  wave_add(&d.ctl[C_NCORE], isc ? 1 : 0);
+  wave_add(&d.ctl[C_DENSE], dense);
+  wave_add(&d.ctl[C_EVAL], t.eval);
+  wave_add(&d.ctl[C_BAND_CORE], t.band);
+  wave_add(&d.ctl[C_BAND], t.band);
+  wave_add(&d.ctl[C_FLIP], t.flip);
+}
+
+// ------------------------------------------------------------- adjacency --
+// Distance (fixed-point units) from a point at fraction f of its cell to
+// the cell o cells away along one axis (0 for o = 0).
+__device__ __forceinline__ float cell_gap(int o, int32_t f) {
+  const int32_t g = o > 0 ? (o << kFixBits) - f : (o < 0 ? f + ((-o - 1) << kFixBits) : 0);
+  return (float)g;
+}
+
+// Light cell, one lane per core point p: for every light stencil cell that
+// can hold a point within eps of p (cell-box distance, the same fixed-point
+// arithmetic as the distance, so never above it) and is not yet known
+// adjacent, test its core points until one is joined to p.
+template <int D, bool F32>
+__device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, int32_t c,
+                         unsigned long long& m0, unsigned long long& m1, Tally& t) {
+  constexpr int NR = D == 3 ? 25 : 5;
+  constexpr int B = Batch<F32>::value;
+  constexpr uint32_t kFrac = (1u << kFixBits) - 1u;
+  uint32_t pg[D];
+  double px[D];
+  load_g<D>(d, p, pg);
+  if (!F32) load_x<D>(d, p, px);
+  int32_t fr[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) fr[k] = (int32_t)(pg[k] & kFrac);
+  const int64_t keyA = (int64_t)d.cell_key[c];
+  const int64_t nkl = I.nk[D - 1];
+  const int64_t nk1 = I.nk[1];
+  const int32_t* rows = d.rows + (int64_t)c * kMaxRows;
+  for (int r = 0; r < NR; ++r) {
+    const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
+    const int o1 = D == 3 ? kRowOff3[r][1] : 0;
+    float g2 = 0.f;
+    {
+      const float a = cell_gap(o0, fr[0]);
+      g2 = a * a;
+      if (D == 3) {
+        const float b = cell_gap(o1, fr[1]);
+        g2 = __builtin_fmaf(b, b, g2);
+      }
+    }
+    if (g2 > P.d2_out) continue;  // the whole line is beyond eps
+    const int32_t pk = rows[r];
+    const int32_t len = pk >> kRowCntShift;
+    if (len == 0) continue;
+    const int32_t st = pk & kRowCellMask;
+    int32_t bnd[6];
+    int64_t key[5];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) bnd[k] = k <= len ? d.cell_start[st + k] : 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) key[k] = k < len ? (int64_t)d.cell_key[st + k] : 0;
+    const int64_t line_delta = D == 3 ? ((int64_t)o0 * nk1 + o1) * nkl : (int64_t)o0 * nkl;
+    int32_t aa[5], ll[5];
+    const int base = 5 * r;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int32_t n = k < len ? bnd[k + 1] - bnd[k] : 0;
+      const int ol = (int)(key[k] - keyA - line_delta);  // offset along the last axis
+      const float e = cell_gap(ol, fr[D - 1]);
+      const bool near = !(__builtin_fmaf(e, e, g2) > P.d2_out);
+      const bool use =
+          n > 0 && n <= kHeavy && st + k != c && near && !mask_has(m0, m1, base + k);
+      aa[k] = bnd[k];
+      ll[k] = use ? n : 0;
+    }
+    Stream5 s;
+    s.set(aa, ll);
+    for (int32_t t0 = 0; t0 < s.tot; t0 += B) {
+      int32_t qi[B];
+      int kk[B];
+      bool cf[B];
+      uint32_t cg[B][D];
+      double cx[B][D];
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        int jr = 0;
+        const bool in = t0 + j < s.tot;
+        qi[j] = in ? s.at(t0 + j, jr) : p;
+        kk[j] = jr;
+        const bool need = in && !mask_has(m0, m1, base + jr);
+        cf[j] = need && d.core[qi[j]];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          if (F32)
+            cg[j][k] = need ? d.G[k][qi[j]] : 0u;
+          else
+            cx[j][k] = need ? d.X[k][qi[j]] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        if (cf[j] && !mask_has(m0, m1, base + kk[j]) &&
+            edge_pair<D, F32>(d, P, p, qi[j], pg, cg[j], px, cx[j], t))
+          mask_set(m0, m1, base + kk[j]);
+      }
+    }
+  }
+}
+
+template <bool F32>
+__global__ __launch_bounds__(256) void k_adj(Dev d) {
+  const int64_t pl = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  bool act = pl < d.n;
+  const int32_t p = act ? (int32_t)pl : 0;
+  int32_t c = 0;
+  int dim = 0;
+  const Inst* I = d.inst;
+  if (act) {
+    c = d.cell_of[p];
+    act = d.core[p] && d.cell_start[c + 1] - d.cell_start[c] <= kHeavy;
+    I = d.inst + d.cell_inst[c];
+    dim = I->dim;
+  }
+  Tally t;
+  unsigned long long m0 = 0, m1 = 0;
+  const Par P(*I);
+  DCLUST_BY_DIM(act, dim, (adj_lane<3, F32>(d, *I, P, p, c, m0, m1, t)),
+                (adj_lane<2, F32>(d, *I, P, p, c, m0, m1, t)));
+  if (m0) atomicOr(&d.amask[2 * (int64_t)c], m0);
+  if (m1) atomicOr(&d.amask[2 * (int64_t)c + 1], m1);
+  wave_add(&d.ctl[C_BAND], t.band);
+  wave_add(&d.ctl[C_ASYM], t.asym);
+}
+
+// Heavy cell A, one wavefront: its core points are the queries (the first
+// kQCache cached in LDS).  Each stencil cell with core points is decided in
+// turn: its core points within eps of the queries' bounding box are the
+// candidates (one per lane), and the queries near the candidates' bounding
+// box are tried until some lane finds a joined pair.
+template <int D, bool F32>
+__device__ void adj_heavy(const Dev& d, const Par& P, int32_t A, int32_t* s_qi, uint32_t* s_qg,
+                          double* s_qx, Tally& t) {
+  constexpr int NR = D == 3 ? 25 : 5;
+  const int lane = lane_id();
+  const int32_t qs = d.cell_start[A], qe = d.cell_start[A + 1];
+  const int32_t ncA = d.cell_ncore[A];
+  if (ncA == 0) return;  // no core point: the mask stays clear
+  const bool allc = ncA == qe - qs;
+  uint32_t blo[D], bhi[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    blo[k] = 0xffffffffu;
+    bhi[k] = 0u;
+  }
+  int nq = 0;
+  for (int32_t i0 = qs; i0 < qe; i0 += 64) {
+    const int32_t i = i0 + lane;
+    const bool v = i < qe && (allc || d.core[i]);
+    uint32_t g[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      g[k] = v ? d.G[k][i] : 0u;
+      blo[k] = v && g[k] < blo[k] ? g[k] : blo[k];
+      bhi[k] = v && g[k] > bhi[k] ? g[k] : bhi[k];
+    }
+    const unsigned long long m = ballot(v);
+    const int pos = nq + mbcnt(m);
+    if (v && pos < kQCache) {
+      s_qi[pos] = i;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        s_qg[pos * 3 + k] = g[k];
+        if (!F32) s_qx[pos * 3 + k] = d.X[k][i];
+      }
+    }
+    nq += __popcll(m);
+  }
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    blo[k] = wave_min(blo[k]);
+    bhi[k] = wave_max(bhi[k]);
+  }
+  wave_sync();
+  const bool cached = nq <= kQCache;
+  const int NQ = cached ? nq : (qe - qs);
+  const int32_t pkl = lane < NR ? d.rows[(int64_t)A * kMaxRows + lane] : 0;
+  unsigned long long m0 = 0, m1 = 0;
+  for (int r = 0; r < NR; ++r) {
+    const int32_t pk = rl_i(pkl, r);
+    const int32_t len = pk >> kRowCntShift, st = pk & kRowCellMask;
+    for (int k = 0; k < len; ++k) {
+      const int32_t Bc = st + k;
+      if (Bc == A) continue;
+      const int32_t b0 = d.cell_start[Bc], b1 = d.cell_start[Bc + 1];
+      if (d.cell_ncore[Bc] == 0) continue;
+      bool edge = false;
+      for (int32_t c0 = b0; c0 < b1 && !edge; c0 += 64) {
+        const int32_t q = c0 + lane;
+        bool cand = q < b1 && d.core[q];
+        uint32_t cg[D];
+        double cx[D];
+#pragma unroll
+        for (int kk = 0; kk < D; ++kk) cg[kk] = cand ? d.G[kk][q] : 0u;
+        cand = cand && !(d2box<D>(cg, blo, bhi) > P.d2_out);
+        if (!ballot(cand)) continue;
+#pragma unroll
+        for (int kk = 0; kk < D; ++kk) cx[kk] = (!F32 && cand) ? d.X[kk][q] : 0.0;
+        uint32_t clo[D], chi[D];
+#pragma unroll
+        for (int kk = 0; kk < D; ++kk) {
+          clo[kk] = wave_min(cand ? cg[kk] : 0xffffffffu);
+          chi[kk] = wave_max(cand ? cg[kk] : 0u);
+        }
+        for (int i = 0; i < NQ && !edge; ++i) {
+          int32_t ui;
+          uint32_t ug[D];
+          double ux[D];
+          if (cached) {
+            ui = s_qi[i];
+#pragma unroll
+            for (int kk = 0; kk < D; ++kk) {
+              ug[kk] = s_qg[i * 3 + kk];
+              ux[kk] = F32 ? 0.0 : s_qx[i * 3 + kk];
+            }
+          } else {
+            ui = qs + i;
+            if (!allc && !d.core[ui]) continue;
+#pragma unroll
+            for (int kk = 0; kk < D; ++kk) {
+              ug[kk] = d.G[kk][ui];
+              ux[kk] = F32 ? 0.0 : d.X[kk][ui];
+            }
+          }
+          if (d2box<D>(ug, clo, chi) > P.d2_out) continue;
+          const bool hit = cand && edge_pair<D, F32>(d, P, ui, q, ug, cg, ux, cx, t);
+          edge = ballot(hit) != 0;
+        }
+      }
+      if (edge) {
+        mask_set(m0, m1, 5 * r + k);
+        if (b1 - b0 <= kHeavy) {
+          // light neighbour: set the bit of A in its mask (A is in its
+          // mirrored stencil line)
+          const int rm = D == 3 ? kRowMirror3[r] : kRowMirror2[r];
+          const int32_t pm = d.rows[(int64_t)Bc * kMaxRows + rm];
+          const int bm = 5 * rm + (A - (pm & kRowCellMask));
+          if (lane == 0) atomicOr(&d.amask[2 * (int64_t)Bc + (bm >> 6)], 1ull << (bm & 63));
+        }
+      }
+    }
+  }
+  if (lane == 0) {
+    d.amask[2 * (int64_t)A] = m0;
+    d.amask[2 * (int64_t)A + 1] = m1;
+  }
+}
+
+template <bool F32>
+__global__ __launch_bounds__(64) void k_adj_heavy(Dev d) {
+  __shared__ int32_t s_qi[kQCache];
+  __shared__ uint32_t s_qg[kQCache * 3];
+  __shared__ double s_qx[F32 ? 1 : kQCache * 3];
+  const long long nh = d.ctl[C_NHEAVY];
+  Tally t;
+  for (long long h = blockIdx.x; h < nh; h += gridDim.x) {
+    const int32_t A = d.heavy[h];
+    const Inst& I = d.inst[d.cell_inst[A]];
+    const Par P(I);
+    if (I.dim == 3)
+      adj_heavy<3, F32>(d, P, A, s_qi, s_qg, s_qx, t);
+    else
+      adj_heavy<2, F32>(d, P, A, s_qi, s_qg, s_qx, t);
+    wave_sync();  // the query cache is reused
+  }
+  wave_add(&d.ctl[C_BAND], t.band);
+  wave_add(&d.ctl[C_ASYM], t.asym);
+}
+
+// ---------------------------------------------------------------- expand --
 // Persistent expansion pass.  Wave w scans its own contiguous segment of
 // the cells for unclaimed cells with core points and starts a new chain at
 // each (device-side seeding; the reference pops host-side seeds,
 // dbscan.cu:142-160); it also drains the correction queue left by the
-// previous pass.  After `budget` pops a wave hands its remaining list to
-// the next pass and exits, so one long chain cannot hold a pass; a wave
-// also exits when its list is empty and no seed source is left.  Every wave
+// previous pass.  A wave pops up to 64 cells of its list at once, one per
+// lane; lane l walks the adjacency bits of its cell: an unclaimed neighbour
+// is claimed for the cell's chain (CAS) and pushed, a neighbour of another
+// chain is a collision pair.  After `budget` cells a wave hands its list to
+// the next pass and exits, so one long chain cannot hold a pass; a wave also
+// exits when its list is empty and no seed source is left.  Every wave
 // reaches the end of the kernel: there is no inter-wave waiting.
-template <bool F32>
+constexpr int kSeen = 256;  // per-wave table of recently recorded collision pairs
+
 __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budget) {
   extern __shared__ int32_t s_lists[];
-  __shared__ int32_t s_x[4][5][kSegMax];
-  __shared__ int32_t s_q[4][64];
+  __shared__ unsigned long long s_seen[4][kSeen];
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   const int nwb = blockDim.x >> 6;
   const int64_t gw = (int64_t)xcd_block(blockIdx.x, gridDim.x) * nwb + wv;
   const int64_t W = (int64_t)gridDim.x * nwb;
   const int32_t C = (int32_t)d.ctl[C_NCELL];
-  ExpLds X{s_x[wv][0], s_x[wv][1], s_x[wv][2], s_x[wv][3], s_x[wv][4], s_q[wv]};
-  SeedList L;
-  L.list = s_lists + wv * S;
-  L.S = S;
-  L.sl = 0;
-  L.qout = d.q[(pass + 1) & 1];
-  L.qout_n = &d.ctl[C_Q0_N + ((pass + 1) & 1)];
-  L.cache = -1;
-  L.cache_pos = 0;
-  L.pcap = d.pair_per;
-  L.pbuf = d.pairs + gw * (int64_t)d.pair_per;
-  L.pcnt = rfl_i(d.pair_cnt[gw]);  // pairs recorded by earlier passes
-  const int pcnt0 = L.pcnt;
-  const unsigned long long t_begin = __builtin_amdgcn_s_memrealtime();
+  int32_t* list = s_lists + wv * S;
+  unsigned long long* seen = s_seen[wv];
+  int32_t* qout = d.q[(pass + 1) & 1];
+  unsigned long long* qout_n = (unsigned long long*)&d.ctl[C_Q0_N + ((pass + 1) & 1)];
   const int32_t* qin = d.q[pass & 1];
-  long long* qin_head = &d.ctl[C_Q0_HEAD + (pass & 1)];
+  unsigned long long* qin_head = (unsigned long long*)&d.ctl[C_Q0_HEAD + (pass & 1)];
   const long long qn = pass > 0 ? d.ctl[C_Q0_N + (pass & 1)] : 0;
   bool qlive = qn > 0;
+  const int pcap = d.pair_per;
+  int2* pbuf = d.pairs + gw * (int64_t)pcap;
+  const int pc0 = rfl_i(d.pair_cnt[gw]);  // pairs recorded by earlier passes
+  int pc = pc0;
+  for (int i = lane; i < kSeen; i += 64) seen[i] = ~0ull;
+  wave_sync();
+  const unsigned long long t_begin = __builtin_amdgcn_s_memrealtime();
   // the seeding cursor over this wave's segment of cells persists across passes
   int64_t seg = d.seg_pos[gw];
   const int64_t seg_end = (int64_t)C * (gw + 1) / W;
-  int pops_left = budget;
-  int32_t cache_own = -2;
-  Tally t;
-  long long ovf = 0;
+  int left = budget;
+  int sl = 0;  // seed-list length (wave-uniform)
+  long long pops = 0, ovf = 0, edges = 0;
   for (;;) {
-    if (pops_left == 0) {
-      if (L.sl > 0) {
+    if (left <= 0) {
+      if (sl > 0) {
         long long base = 0;
-        if (lane == 0) base = (long long)atomicAdd((unsigned long long*)L.qout_n, (unsigned long long)L.sl);
+        if (lane == 0) base = (long long)atomicAdd(qout_n, (unsigned long long)sl);
         base = __shfl(base, 0, 64);
-        for (int i = lane; i < L.sl; i += 64) L.qout[base + i] = L.list[i];
-        L.sl = 0;
+        for (int i = lane; i < sl; i += 64) qout[base + i] = list[i];
       }
       break;
     }
-    if (L.sl == 0) {
+    if (sl == 0) {
       bool got = false;
       if (qlive) {
+        const int take = S < 64 ? S : 64;  // a refill fits the seed list
         long long base = 0;
-        if (lane == 0) base = (long long)atomicAdd((unsigned long long*)qin_head, 64ull);
+        if (lane == 0) base = (long long)atomicAdd(qin_head, (unsigned long long)take);
         base = __shfl(base, 0, 64);
         if (base < qn) {
-          const int cnt = (int)((qn - base) < 64 ? (qn - base) : 64);
-          if (lane < cnt) L.list[lane] = qin[base + lane];
-          L.sl = cnt;
+          const int cnt = (int)((qn - base) < take ? (qn - base) : take);
+          if (lane < cnt) list[lane] = qin[base + lane];
+          sl = cnt;
           got = true;
         } else {
           qlive = false;
@@ -687,8 +745,8 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
           }
           ok = __shfl(ok, 0, 64);
           if (ok) {
-            if (lane == 0) L.list[0] = (int32_t)(seg + l);
-            L.sl = 1;
+            if (lane == 0) list[0] = (int32_t)(seg + l);
+            sl = 1;
             got = true;
             seg = seg + l + 1;
             break;
@@ -700,45 +758,116 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
       if (!got) break;
       wave_sync();
     }
-    // pop one cell
-    const int32_t A = L.list[L.sl - 1];
-    L.sl -= 1;
-    wave_sync();
-    const int32_t own = d.cell_label[A];
-    if (own < 0) continue;  // cannot happen: list entries are claimed cells
-    pops_left--;
-    t.pops++;
-    t.batch += 1;
-    if (own != cache_own) {
-      L.cache = -1;
-      L.cache_pos = 0;
-      cache_own = own;
+    // pop the top P cells, one per lane
+    const int P = sl < 64 ? (sl < left ? sl : left) : (64 < left ? 64 : left);
+    const bool act = lane < P;
+    const int32_t A = act ? list[sl - P + lane] : 0;
+    wave_sync();  // popped before the pushes below overwrite the slots
+    sl -= P;
+    left -= P;
+    pops += P;
+    int32_t own = 0;
+    unsigned long long m0 = 0, m1 = 0;
+    if (act) {
+      // coherent read: the claim may have been made moments ago on another
+      // XCD, whose L2 is not coherent with this one for plain loads
+      own = __hip_atomic_load(&d.cell_label[A], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      m0 = d.amask[2 * (int64_t)A];
+      m1 = d.amask[2 * (int64_t)A + 1];
     }
-    const Inst& I = d.inst[d.cell_inst[A]];
-    if (I.dim == 3)
-      expand_cell<3, F32>(d, I, A, own, L, X, t, ovf);
-    else
-      expand_cell<2, F32>(d, I, A, own, L, X, t, ovf);
+    edges += __popcll(m0) + __popcll(m1);
+    const int32_t* rowsA = d.rows + (int64_t)A * kMaxRows;
+    while (ballot((m0 | m1) != 0ull)) {
+      constexpr int T = 8;  // neighbours per lane step
+      int32_t tg[T];
+      bool v[T];
+#pragma unroll
+      for (int j = 0; j < T; ++j) {
+        int b = 0;
+        v[j] = true;
+        if (m0) {
+          b = __ffsll((long long)m0) - 1;
+          m0 &= m0 - 1;
+        } else if (m1) {
+          b = 64 + __ffsll((long long)m1) - 1;
+          m1 &= m1 - 1;
+        } else {
+          v[j] = false;
+        }
+        const int r = (b * 205) >> 10;  // b / 5 for 0 <= b < 128
+        tg[j] = v[j] ? (rowsA[r] & kRowCellMask) + (b - 5 * r) : 0;
+      }
+      // a plain load can only be stale as kUnclaimed (a label is written
+      // once), so only those cells pay for a CAS
+      int32_t lab[T];
+#pragma unroll
+      for (int j = 0; j < T; ++j) lab[j] = v[j] ? d.cell_label[tg[j]] : own;
+      bool cl[T];
+#pragma unroll
+      for (int j = 0; j < T; ++j) {
+        cl[j] = false;
+        if (v[j] && lab[j] == kUnclaimed) {
+          const int32_t old = atomicCAS(&d.cell_label[tg[j]], kUnclaimed, own);
+          cl[j] = old == kUnclaimed;
+          lab[j] = cl[j] ? own : old;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < T; ++j) {
+        // pushes: one slot range per step
+        const unsigned long long mc = ballot(cl[j]);
+        if (mc) {
+          const int pos = sl + mbcnt(mc);
+          if (cl[j]) {
+            if (pos < S) {
+              list[pos] = tg[j];
+            } else {
+              const unsigned long long qi = atomicAdd(qout_n, 1ull);
+              qout[qi] = tg[j];
+              ++ovf;
+            }
+          }
+          sl += __popcll(mc);
+          sl = sl < S ? sl : S;
+        }
+        // collisions: each distinct (own, other) pair of the step once
+        bool coll = v[j] && lab[j] != own;
+        unsigned long long cm = ballot(coll);
+        while (cm) {
+          const int l = __ffsll((long long)cm) - 1;
+          const int32_t o = rl_i(own, l), f = rl_i(lab[j], l);
+          const unsigned long long key = ((unsigned long long)(uint32_t)o << 32) | (uint32_t)f;
+          const int h = (int)(((uint32_t)o * 0x9E3779B1u ^ (uint32_t)f * 0x85EBCA6Bu) >> 24);
+          if (seen[h] != key) {
+            if (lane == 0) {
+              seen[h] = key;
+              if (pc < pcap)
+                pbuf[pc] = make_int2(o, f);
+              else
+                uf_union(d.parent, o, f);  // buffer full: merge inline
+            }
+            ++pc;
+          }
+          coll = coll && !(own == o && lab[j] == f);
+          cm = ballot(coll);
+        }
+      }
+    }
+    wave_sync();
   }
-  if (lane_id() == 0) {
+  if (lane == 0) {
     // 100 MHz constant clock -> microseconds of this wave's life
     const unsigned long long us = (__builtin_amdgcn_s_memrealtime() - t_begin) / 100ull;
-    d.pair_cnt[gw] = L.pcnt;
+    d.pair_cnt[gw] = pc;
     d.seg_pos[gw] = seg;
     if (seg < seg_end) atomicAdd((unsigned long long*)&d.ctl[C_SEGS_LEFT], 1ull);
-    if (L.pcnt > pcnt0)
-      atomicAdd((unsigned long long*)&d.ctl[C_NPAIR], (unsigned long long)(L.pcnt - pcnt0));
+    if (pc > pc0) atomicAdd((unsigned long long*)&d.ctl[C_NPAIR], (unsigned long long)(pc - pc0));
     atomicAdd((unsigned long long*)&d.ctl[C_BUSY_SUM], us);
     atomicMax((unsigned long long*)&d.ctl[C_BUSY_MAX], us);
-    if (ovf) atomicAdd((unsigned long long*)&d.ctl[C_OVF], (unsigned long long)ovf);
-    atomicAdd((unsigned long long*)&d.ctl[C_POPS], (unsigned long long)t.pops);
-    atomicAdd((unsigned long long*)&d.ctl[C_BATCH], (unsigned long long)t.batch);
-    atomicAdd((unsigned long long*)&d.ctl[C_XCHUNK], (unsigned long long)t.xchunk);
-    atomicAdd((unsigned long long*)&d.ctl[C_XREL], (unsigned long long)t.xrel);
+    atomicAdd((unsigned long long*)&d.ctl[C_POPS], (unsigned long long)pops);
   }
-  wave_add(&d.ctl[C_BAND], t.band);
-  wave_add(&d.ctl[C_FLIP], t.flip);
-  wave_add(&d.ctl[C_ASYM], t.asym);
+  wave_add(&d.ctl[C_OVF], ovf);
+  wave_add(&d.ctl[C_EDGES], edges);
 }
 
 // Resets the counters of the queue that pass `pass` fills and of the head
@@ -892,120 +1021,106 @@ __global__ __launch_bounds__(256) void k_keys(Dev d) {
     }
   }
 }
-
 // ---------------------------------------------------------------- border --
-// Non-core points of a cell (queries) take the minimum component key over
-// the core candidates q of the stencil (own cell included) with
-// P(q -> query), evaluated from the core point's side as its expansion in
-// the reference oracle would.
+// P(q -> p) for a core point q and a non-core point p, evaluated from the
+// core point's side as its expansion in the reference oracle would.
 template <int D, bool F32>
-__device__ void border_cell(const Dev& d, const Inst& I, int32_t c, int32_t* pref, int32_t* off,
-                            Tally& t) {
-  constexpr int NR = D == 3 ? 25 : 5;
-  const int lane = lane_id();
-  const int32_t qs = d.cell_start[c], qe = d.cell_start[c + 1];
-  if (d.cell_ncore[c] == qe - qs) return;  // every point is core
-  Seg st;
-  stencil_points<NR>(d, c, false, pref, off, st);
-  for (int32_t q0 = qs; q0 < qe; q0 += 64) {
-    const int nq = (qe - q0) < 64 ? (qe - q0) : 64;
-    const bool isq = lane < nq && !d.core[q0 + lane];
-    unsigned long long qm = ballot(isq);
-    if (!qm) continue;
-    double qx[D];
+__device__ __forceinline__ bool pred_to(const Dev& d, const Par& P, int32_t q, int32_t p,
+                                        const uint32_t* pg, const double* px, Tally& t) {
+  double qx[D];
+  if (F32) {
     uint32_t qg[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      qx[k] = (!F32 && isq) ? d.X[k][q0 + lane] : 0.0;
-      qg[k] = (F32 && isq) ? d.G[k][q0 + lane] : 0u;
-    }
-    int32_t best = kNoKey;
-    for (int32_t m0 = 0; m0 < st.M; m0 += 64) {
-      const int32_t m = m0 + lane;
-      const bool valid = m < st.M;
-      int sk;
-      const int32_t pt = valid ? seg_at(st, m, sk) : 0;
-      const bool cc = valid && d.core[pt];
-      if (!ballot(cc)) continue;
-      double cx[D], clo[D], chi[D];
-      uint32_t cg[D];
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        cx[k] = (cc && !F32) ? d.X[k][pt] : 0.0;
-        clo[k] = cx[k] - I.eps;
-        chi[k] = cx[k] + I.eps;
-        cg[k] = (F32 && cc) ? d.G[k][pt] : 0u;
-      }
-      const int32_t ck = cc ? d.keyv[pt] : kNoKey;
-      unsigned long long todo = qm;
-      while (todo) {
-        const int qi = __ffsll((long long)todo) - 1;
-        todo &= todo - 1;
-        bool hit;
-        if (F32) {
-          uint32_t ug[D];
-#pragma unroll
-          for (int k = 0; k < D; ++k) ug[k] = rl_u(qg[k], qi);
-          float s = d2fix<D>(cg, ug);
-          bool in = s < I.d2_in;
-          bool amb = cc && !in && s <= I.d2_out;
-          if (ballot(amb)) {
-            double ux[D];
-#pragma unroll
-            for (int k = 0; k < D; ++k) ux[k] = d.X[k][q0 + qi];
-            if (amb) {
-              double px[D], plo[D], phi[D];
-#pragma unroll
-              for (int k = 0; k < D; ++k) {
-                px[k] = d.X[k][pt];
-                plo[k] = px[k] - I.eps;
-                phi[k] = px[k] + I.eps;
-              }
-              bool ex = pred64<D>(px, plo, phi, ux, I.eps2);
-              t.band++;
-              t.flip += (ex != (s <= I.d2_mid));
-              in = ex;
-            }
-          }
-          hit = cc && in;
-        } else {
-          double ux[D];
-#pragma unroll
-          for (int k = 0; k < D; ++k) ux[k] = rl_d(qx[k], qi);
-          hit = cc && pred64<D>(cx, clo, chi, ux, I.eps2);
-        }
-        if (ballot(hit)) {
-          const int32_t mn = wave_min(hit ? ck : kNoKey);
-          if (lane == qi && mn < best) best = mn;
-        }
-      }
-    }
-    if (isq) d.keyv[q0 + lane] = best;
+    load_g<D>(d, q, qg);
+    const float s = d2fix<D>(qg, pg);
+    if (s < P.d2_in) return true;
+    if (s > P.d2_out) return false;
+    t.band++;
+    double ax[D];
+    load_x<D>(d, q, qx);
+    load_x<D>(d, p, ax);
+    const bool ex = box_64<D>(qx, ax, P.eps) && d2_64<D>(qx, ax) <= P.eps2;
+    t.flip += ex != (s <= P.d2_mid);
+    return ex;
   }
+  load_x<D>(d, q, qx);
+  return box_64<D>(qx, px, P.eps) && d2_64<D>(qx, px) <= P.eps2;
+}
+
+// Smallest component key over the core points q with P(q -> p): the own
+// cell (a clique), then the recorded neighbours, or the whole stencil when
+// the record overflowed or is absent.
+template <int D, bool F32>
+__device__ int32_t border_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int32_t qs,
+                               int32_t qe, int R, Tally& t, long long& ovf) {
+  constexpr int NR = D == 3 ? 25 : 5;
+  int32_t best = kNoKey;
+  for (int32_t q = qs; q < qe; ++q)
+    if (d.core[q]) {
+      const int32_t k = d.keyv[q];
+      best = k < best ? k : best;
+    }
+  uint32_t pg[D];
+  double px[D];
+  if (F32)
+    load_g<D>(d, p, pg);
+  else
+    load_x<D>(d, p, px);
+  const int n = R > 0 ? (int)d.rec_n[p] : 256;
+  if (n <= R) {
+    const int32_t* rec = d.rec + (int64_t)p * R;
+    for (int j = 0; j < n; ++j) {
+      const int32_t q = rec[j];
+      if (!d.core[q]) continue;
+      const int32_t k = d.keyv[q];
+      if (k < best && pred_to<D, F32>(d, P, q, p, pg, px, t)) best = k;
+    }
+    return best;
+  }
+  if (R > 0) ++ovf;
+  const int32_t* rows = d.rows + (int64_t)c * kMaxRows;
+  for (int g = 0; g < NR; g += 5) {
+    int32_t aa[5], ll[5];
+    line_group(d, rows + g, aa, ll);
+    Stream5 s;
+    s.set(aa, ll);
+    for (int32_t i = 0; i < s.tot; ++i) {
+      int jr;
+      const int32_t q = s.at(i, jr);
+      if ((q >= qs && q < qe) || !d.core[q]) continue;
+      const int32_t k = d.keyv[q];
+      if (k < best && pred_to<D, F32>(d, P, q, p, pg, px, t)) best = k;
+    }
+  }
+  return best;
 }
 
 template <bool F32>
 __global__ __launch_bounds__(256) void k_border(Dev d) {
-  __shared__ int32_t s_seg[4][2][kSegMax];
-  const int wv = threadIdx.x >> 6;
-  const int32_t C = (int32_t)d.ctl[C_NCELL];
-  const int nwb = blockDim.x >> 6;
-  const int64_t gw = (int64_t)xcd_block(blockIdx.x, gridDim.x) * nwb + wv;
-  const int64_t W = (int64_t)gridDim.x * nwb;
-  constexpr int CH = 16;
-  Tally t;
-  for (int64_t c0 = gw * CH; c0 < C; c0 += W * CH) {
-    const int32_t ce = (int32_t)((c0 + CH) < C ? (c0 + CH) : C);
-    for (int32_t c = (int32_t)c0; c < ce; ++c) {
-      const Inst& I = d.inst[d.cell_inst[c]];
-      if (I.dim == 3)
-        border_cell<3, F32>(d, I, c, s_seg[wv][0], s_seg[wv][1], t);
-      else
-        border_cell<2, F32>(d, I, c, s_seg[wv][0], s_seg[wv][1], t);
-    }
+  const int64_t pl = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  bool act = pl < d.n;
+  const int32_t p = act ? (int32_t)pl : 0;
+  int32_t c = 0, qs = 0, qe = 0;
+  int dim = 0;
+  const Inst* I = d.inst;
+  if (act) {
+    act = !d.core[p];
+    c = d.cell_of[p];
+    qs = d.cell_start[c];
+    qe = d.cell_start[c + 1];
+    I = d.inst + d.cell_inst[c];
+    dim = I->dim;
   }
+  Tally t;
+  long long ovf = 0;
+  int32_t best = kNoKey;
+  const int R = d.rec_stride;
+  const Par P(*I);
+  DCLUST_BY_DIM(act, dim, (best = border_lane<3, F32>(d, P, p, c, qs, qe, R, t, ovf)),
+                (best = border_lane<2, F32>(d, P, p, c, qs, qe, R, t, ovf)));
+  if (act) d.keyv[p] = best;
   wave_add(&d.ctl[C_BAND], t.band);
   wave_add(&d.ctl[C_FLIP], t.flip);
+  wave_add(&d.ctl[C_RECOVF], ovf);
 }
 
 // ------------------------------------------------------------ rank + out --
@@ -1073,11 +1188,28 @@ __global__ __launch_bounds__(256) void k_output(Dev d) {
 }
 
 // ------------------------------------------------------------- launchers --
-void launch_core(const Dev& d, bool f32, int grid, hipStream_t s) {
+// one lane per point, grid a multiple of 8 (XCD remap)
+static int point_grid(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  b = (b + 7) / 8 * 8;
+  return (int)(b < 8 ? 8 : b);
+}
+
+void launch_core(const Dev& d, bool f32, hipStream_t s) {
   if (f32)
-    hipLaunchKernelGGL(k_core<true>, dim3(grid), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_core<true>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
   else
-    hipLaunchKernelGGL(k_core<false>, dim3(grid), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_core<false>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
+}
+
+void launch_adj(const Dev& d, bool f32, int heavy_grid, hipStream_t s) {
+  if (f32) {
+    hipLaunchKernelGGL(k_adj<true>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_adj_heavy<true>, dim3(heavy_grid), dim3(64), 0, s, d);
+  } else {
+    hipLaunchKernelGGL(k_adj<false>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_adj_heavy<false>, dim3(heavy_grid), dim3(64), 0, s, d);
+  }
 }
 
 void launch_pass_prep(const Dev& d, int pass, hipStream_t s) {
@@ -1092,20 +1224,16 @@ void launch_seg_init(const Dev& d, int64_t waves, hipStream_t s) {
   hipLaunchKernelGGL(k_seg_init, dim3(64), dim3(256), 0, s, d, waves);
 }
 
-void launch_expand(const Dev& d, bool f32, int pass, int grid, int seeds, int threads,
-                   int budget, hipStream_t s) {
+void launch_expand(const Dev& d, int pass, int grid, int seeds, int threads, int budget,
+                   hipStream_t s) {
   size_t lds = (size_t)(threads / 64) * seeds * sizeof(int32_t);
-  if (f32)
-    hipLaunchKernelGGL(k_expand<true>, dim3(grid), dim3(threads), lds, s, d, pass, seeds, budget);
-  else
-    hipLaunchKernelGGL(k_expand<false>, dim3(grid), dim3(threads), lds, s, d, pass, seeds, budget);
+  hipLaunchKernelGGL(k_expand, dim3(grid), dim3(threads), lds, s, d, pass, seeds, budget);
 }
 
-int expand_blocks_per_cu(bool f32, int seeds, int threads) {
+int expand_blocks_per_cu(int seeds, int threads) {
   int nb = 0;
   size_t lds = (size_t)(threads / 64) * seeds * sizeof(int32_t);
-  hipError_t e = f32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_expand<true>, threads, lds)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_expand<false>, threads, lds);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_expand, threads, lds);
   if (e != hipSuccess || nb < 1) nb = 1;
   return nb;
 }
@@ -1116,16 +1244,16 @@ void launch_merge(const Dev& d, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_flatten, dim3(grid), dim3(256), 0, s, d);
 }
 
-void launch_border(const Dev& d, bool f32, int grid, int64_t n, hipStream_t s) {
-  int g2 = (int)((n + 255) / 256);
+void launch_border(const Dev& d, bool f32, hipStream_t s) {
+  int g2 = (int)((d.n + 255) / 256);
   if (g2 > 4096) g2 = 4096;
   if (g2 < 1) g2 = 1;
   hipLaunchKernelGGL(k_compmin, dim3(g2), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_keys, dim3(g2), dim3(256), 0, s, d);
   if (f32)
-    hipLaunchKernelGGL(k_border<true>, dim3(grid), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_border<true>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
   else
-    hipLaunchKernelGGL(k_border<false>, dim3(grid), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_border<false>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
 }
 
 void launch_relabel(const Dev& d, int64_t n, hipStream_t s) {

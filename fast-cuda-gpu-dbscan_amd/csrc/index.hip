@@ -640,14 +640,6 @@ __global__ __launch_bounds__(256) void k_line_mark(Dev d) {
   }
 }
 
-// Neighbour lines of the 5^D stencil, nearest first: (0,0), the 4 at
-// distance 1, the 4 diagonals, then the ring at distance 2.
-__constant__ int8_t kRowOff3[25][2] = {
-    {0, 0},   {-1, 0}, {1, 0},  {0, -1}, {0, 1},  {-1, -1}, {-1, 1}, {1, -1}, {1, 1},
-    {-2, 0},  {2, 0},  {0, -2}, {0, 2},  {-2, -1}, {-2, 1}, {2, -1}, {2, 1},  {-1, -2},
-    {1, -2},  {-1, 2}, {1, 2},  {-2, -2}, {-2, 2}, {2, -2}, {2, 2}};
-__constant__ int8_t kRowOff2[5] = {0, -1, 1, -2, 2};
-
 // For each (cell, stencil line): the run of non-empty cells of that line
 // whose last coordinate is within +-2 of the cell's, packed as
 // first | count << 28 (count <= 5).  One thread per (cell, line).  The

@@ -4,12 +4,14 @@
 // largest run seen so far and reused):
 //   input   xyz       caller's row-major FP64 rows, instances back to back
 //   sort    key[2]    u64 cell keys (ping-pong), val[2] i32 point ids
-//   CSR     cell_start[C+1], cell_key[C], cell_inst[C], rows[C][9] (int2
-//           point ranges of the 3^(D-1) neighbour rows, centre row first)
+//   CSR     cell_start[C+1], cell_key[C], cell_inst[C], rows[C][25] (packed
+//           runs of the 5^(D-1) neighbour lines, centre line first),
+//           cell_ncore[C], cell_label[C] (chain), amask[C][2] (adjacency)
 //   points  (cell-sorted)  X[3][N] FP64 SoA, G[3][N] u32 fixed-point
-//           coordinates (30 fraction bits per cell, wrapping mod 4 cells),
-//           orig[N], cell_of[N], core[N] u8, label[N] (chain id)
-//   merge   parent[N] chain union-find, compmin[N], pairs[P] int2
+//           coordinates (29 fraction bits per cell, wrapping mod 8 cells),
+//           orig[N], cell_of[N], core[N] u8, rec[N][R] + rec_n[N] (the
+//           neighbours of each non-core point)
+//   merge   parent[] chain union-find, compmin[N], pairs[waves][E] int2
 //   output  keyv[N] (component min input index), flag[N] u8, rank[N]
 #pragma once
 #include <hip/hip_runtime.h>
@@ -26,6 +28,40 @@ constexpr int kMaxRows = 25;  // 5^(D-1) neighbour lines for D <= 3
 constexpr int kRowCntShift = 28;  // packed row: first cell | cell count << 28
 constexpr int32_t kRowCellMask = (1 << kRowCntShift) - 1;
 constexpr int64_t kMaxCells = 1LL << kRowCntShift;
+constexpr int kHeavy = 32;    // cells with more points are "heavy" (wavefront paths)
+constexpr int kMaxRec = 64;   // largest minpts for which neighbour records are kept
+
+// Stencil lines, nearest first: (offset along axis 0, offset along axis 1)
+// of line r for D = 3, offset along axis 0 for D = 2.  The mirror of line r
+// is the line with the negated offsets (used to set the adjacency bit of
+// the other cell of a pair).
+#define DCLUST_ROW_OFF3                                                                     \
+  {{0, 0},   {-1, 0}, {1, 0},   {0, -1},  {0, 1},  {-1, -1}, {-1, 1}, {1, -1}, {1, 1},       \
+   {-2, 0},  {2, 0},  {0, -2},  {0, 2},   {-2, -1}, {-2, 1}, {2, -1}, {2, 1},  {-1, -2},     \
+   {1, -2},  {-1, 2}, {1, 2},   {-2, -2}, {-2, 2}, {2, -2}, {2, 2}}
+#define DCLUST_ROW_MIRROR3 \
+  {0, 2, 1, 4, 3, 8, 7, 6, 5, 10, 9, 12, 11, 16, 15, 14, 13, 20, 19, 18, 17, 24, 23, 22, 21}
+#define DCLUST_ROW_OFF2 {0, -1, 1, -2, 2}
+#define DCLUST_ROW_MIRROR2 {0, 2, 1, 4, 3}
+static __constant__ int8_t kRowOff3[25][2] = DCLUST_ROW_OFF3;
+static __constant__ int8_t kRowOff2[5] = DCLUST_ROW_OFF2;
+static __constant__ int8_t kRowMirror3[25] = DCLUST_ROW_MIRROR3;
+static __constant__ int8_t kRowMirror2[5] = DCLUST_ROW_MIRROR2;
+
+namespace tables {
+constexpr int8_t off3[25][2] = DCLUST_ROW_OFF3;
+constexpr int8_t mir3[25] = DCLUST_ROW_MIRROR3;
+constexpr int8_t off2[5] = DCLUST_ROW_OFF2;
+constexpr int8_t mir2[5] = DCLUST_ROW_MIRROR2;
+constexpr bool mirrors_ok() {
+  for (int r = 0; r < 25; ++r)
+    if (off3[mir3[r]][0] != -off3[r][0] || off3[mir3[r]][1] != -off3[r][1]) return false;
+  for (int r = 0; r < 5; ++r)
+    if (off2[mir2[r]] != -off2[r]) return false;
+  return true;
+}
+static_assert(mirrors_ok(), "stencil line mirror table");
+}  // namespace tables
 
 // Per-instance parameters and derived grid (device table, one per instance).
 struct Inst {
@@ -81,17 +117,16 @@ enum Ctl : int {
   C_ERR,
   C_SORTPAR,  // number of executed radix scatters (buffer parity)
   C_BAND_CORE,
-  C_CANDLOAD,
-  C_POPS,
-  C_BATCH,
-  C_XCHUNK,
-  C_XREL,
+  C_POPS,       // cells expanded
   C_BUSY_SUM,  // expansion: sum over waves of their lifetime (us)
   C_BUSY_MAX,
   C_SEGS_LEFT,  // expansion waves whose seeding segment is not finished
   C_NLINES,     // total line-table entries
   C_DENSE,      // cells resolved as all-core without distance tests
-  C_NCTL = 32
+  C_NHEAVY,     // heavy cells (listed in Dev::heavy)
+  C_EDGES,      // adjacency bits of the expanded cells
+  C_RECOVF,     // non-core points whose neighbour record overflowed
+  C_NCTL = 36
 };
 
 // Sort-pass bookkeeping: pass p is active when p*8 < key bits; its scatter
@@ -124,6 +159,8 @@ struct Dev {
   int32_t* cell_inst;   // [N]
   int32_t* cell_ncore;  // [N] core points per cell
   int32_t* cell_label;  // [N] chain owning the cell's core points
+  unsigned long long* amask;  // [N][2] adjacency: bit 5r+k <-> k-th cell of line r
+  int32_t* heavy;       // heavy cells (more than kHeavy points)
   int32_t* rows;        // [N][kMaxRows] packed neighbour lines (first cell, count)
   int32_t* cell_of;     // [N] per sorted point
   int2* lines;          // [line_cap] first / end cell of every possible line
@@ -134,7 +171,9 @@ struct Dev {
   uint32_t* G[3];
   int32_t* orig;
   uint8_t* core;
-  int32_t* label;
+  int32_t* rec;         // [N][rec_stride] neighbours of non-core points
+  uint8_t* rec_n;       // [N] records written (saturates at 255)
+  int32_t rec_stride;   // 0: no records (minpts > kMaxRec)
   int32_t* parent;
   int32_t* compmin;
   int2* pairs;          // [waves][pair_per] collision pairs per expansion wave

@@ -22,17 +22,18 @@ void launch_scan_top(int64_t* v, int nb, hipStream_t s);
 int64_t sort_hist_entries(int64_t n);
 int64_t scan_tmp_entries(int64_t n);
 
-void launch_core(const Dev& d, bool f32, int grid, hipStream_t s);
+void launch_core(const Dev& d, bool f32, hipStream_t s);
+void launch_adj(const Dev& d, bool f32, int heavy_grid, hipStream_t s);
 void launch_pass_prep(const Dev& d, int pass, hipStream_t s);
 void launch_pass_carry(const Dev& d, int pass, hipStream_t s);
 void launch_seg_init(const Dev& d, int64_t waves, hipStream_t s);
-void launch_expand(const Dev& d, bool f32, int pass, int grid, int seeds,
-                   int threads, int budget, hipStream_t s);
+void launch_expand(const Dev& d, int pass, int grid, int seeds, int threads, int budget,
+                   hipStream_t s);
 void launch_merge(const Dev& d, int grid, hipStream_t s);
-void launch_border(const Dev& d, bool f32, int grid, int64_t n, hipStream_t s);
+void launch_border(const Dev& d, bool f32, hipStream_t s);
 void launch_relabel(const Dev& d, int64_t n, hipStream_t s);
 
 // occupancy helper for the expansion kernel
-int expand_blocks_per_cu(bool f32, int seeds, int threads);
+int expand_blocks_per_cu(int seeds, int threads);
 
 }  // namespace dclust

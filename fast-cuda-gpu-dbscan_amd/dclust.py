@@ -51,12 +51,12 @@ class Params(C.Structure):
     ]
 
 
-_STAT_F64 = ["t_total_ms", "t_index_ms", "t_core_ms", "t_expand_ms", "t_merge_ms",
+_STAT_F64 = ["t_total_ms", "t_index_ms", "t_core_ms", "t_adj_ms", "t_expand_ms", "t_merge_ms",
              "t_border_ms", "t_relabel_ms"]
 _STAT_I64 = ["n_points", "n_cells", "n_core", "n_noise", "n_clusters", "n_chains",
              "n_collisions", "n_overflow", "n_passes", "cand_total", "nbr_cells_total",
              "band_rechecks", "band_flips", "asym_pairs", "pairs_evaluated",
-             "band_rechecks_core", "core_cand_loads"]
+             "band_rechecks_core", "rec_overflow"]
 _STAT_I32 = ["hip_error", "key_bits"]
 
 

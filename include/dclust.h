@@ -80,6 +80,7 @@ typedef struct dclust_stats {
   double t_total_ms;   /* first kernel to labels resident on the device   */
   double t_index_ms;   /* bbox + cell keys + radix sort + CSR + rows        */
   double t_core_ms;    /* neighbour-count (core flag) kernel                */
+  double t_adj_ms;     /* cell adjacency (core-core pairs within eps)       */
   double t_expand_ms;  /* seed-list expansion passes                        */
   double t_merge_ms;   /* union-find merge of collision pairs               */
   double t_border_ms;  /* border assignment                                 */
@@ -100,14 +101,15 @@ typedef struct dclust_stats {
   int64_t asym_pairs;      /* core-core pairs with P(p,q) != P(q,p)         */
   int64_t pairs_evaluated; /* distance evaluations in the core kernel       */
   int64_t band_rechecks_core; /* guard-band rechecks inside the core kernel */
-  int64_t core_cand_loads;    /* candidate points loaded by the core kernel */
+  int64_t rec_overflow;       /* non-core points whose neighbour record
+                                 * overflowed (their border is found by a scan) */
   int32_t hip_error;       /* last hipError_t, 0 if none                    */
   int32_t key_bits;        /* bits of the sort key                          */
   /* diagnostics: [0] cells expanded, [1] cells resolved as all-core without
-   * distance tests, [2] expansions, [3] expansions that tested candidate
-   * cells, [4] time of expansion pass 0 (us), [5] sum over expansion
-   * wavefronts of their lifetime (us), [6] longest lifetime (us),
-   * [7] expansion wavefronts */
+   * distance tests, [2] heavy cells (adjacency by one wavefront per cell),
+   * [3] adjacent cell pairs (directed), [4] time of expansion pass 0 (us),
+   * [5] sum over expansion wavefronts of their lifetime (us), [6] longest
+   * lifetime (us), [7] expansion wavefronts */
   int64_t diag[8];
 } dclust_stats;
 
