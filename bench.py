@@ -172,7 +172,7 @@ def main():
     value = n * args.steps / dt / 1e6
     t_core = float(np.mean([s["t_core_ms"] for s in sts]))
     phases = {k: round(float(np.mean([s[k] for s in sts])), 4)
-              for k in ("t_index_ms", "t_core_ms", "t_adj_ms", "t_expand_ms", "t_merge_ms",
+              for k in ("t_index_ms", "t_core_ms", "t_adj_ms", "t_expand_ms", "t_merge_ms", "t_far_ms",
                         "t_border_ms", "t_relabel_ms", "t_total_ms")}
     bpc = 4 if prec == dclust.FP32_GUARD else 8
     bytes_core = b_nbr(st, dim, bpc)

@@ -33,7 +33,10 @@ struct Buf {
   size_t cap = 0;
 };
 
-enum Ev { E_START, E_INDEX, E_CORE0, E_CORE, E_ADJ, E_PASS0, E_EXPAND, E_MERGE, E_BORDER, E_END, E_N };
+enum Ev {
+  E_START, E_INDEX, E_CORE0, E_CORE, E_ADJ, E_PASS0, E_EXPAND, E_MERGE, E_FAR, E_BORDER, E_END,
+  E_N
+};
 
 }  // namespace
 
@@ -81,7 +84,7 @@ enum Slot {
   S_DTOT, S_PASS, S_CSTART, S_CKEY, S_CINST, S_CM, S_ROWS, S_CELLOF, S_X0, S_X1, S_X2,
   S_G0, S_G1, S_G2, S_ORIG, S_CORE, S_LABEL, S_PARENT, S_COMPMIN, S_PAIRS, S_Q0, S_Q1,
   S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT, S_PAIRCNT, S_SEGPOS, S_LINES, S_AMASK,
-  S_HEAVY, S_REC, S_RECN
+  S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP
 };
 
 // line table entries (8 bytes each) above which neighbour lines are found
@@ -243,6 +246,8 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     d.pass_scatter = h->get<int32_t>(S_PASS, kMaxPasses, &E);
     d.cell_start = h->get<int32_t>(S_CSTART, ntot + 1, &E);
     d.cell_key = h->get<uint64_t>(S_CKEY, ntot, &E);
+    d.cell_xyz = h->get<uint64_t>(S_CXYZ, ntot, &E);
+    d.cell_comp = h->get<int32_t>(S_CCOMP, ntot, &E);
     d.cell_inst = h->get<int32_t>(S_CINST, ntot, &E);
     d.cell_ncore = h->get<int32_t>(S_CM, ntot, &E);
     d.rows = h->get<int32_t>(S_ROWS, (size_t)ntot * kMaxRows, &E);
@@ -343,6 +348,8 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     HIPCHK(hipEventRecord(h->ev[E_EXPAND], s));
     launch_merge(d, grid_cells, s);
     HIPCHK(hipEventRecord(h->ev[E_MERGE], s));
+    launch_far(d, f32, grid_cells, s);
+    HIPCHK(hipEventRecord(h->ev[E_FAR], s));
     launch_border(d, f32, s);
     HIPCHK(hipEventRecord(h->ev[E_BORDER], s));
     launch_relabel(d, ntot, s);
@@ -359,7 +366,8 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
       stats->t_adj_ms = ev_ms(h, E_CORE, E_ADJ);
       stats->t_expand_ms = ev_ms(h, E_ADJ, E_EXPAND);
       stats->t_merge_ms = ev_ms(h, E_EXPAND, E_MERGE);
-      stats->t_border_ms = ev_ms(h, E_MERGE, E_BORDER);
+      stats->t_far_ms = ev_ms(h, E_MERGE, E_FAR);
+      stats->t_border_ms = ev_ms(h, E_FAR, E_BORDER);
       stats->t_relabel_ms = ev_ms(h, E_BORDER, E_END);
       stats->t_total_ms = ev_ms(h, E_START, E_END);
       stats->n_cells = ctl_h[C_NCELL];
@@ -386,6 +394,10 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
       stats->diag[5] = ctl_h[C_BUSY_SUM];
       stats->diag[6] = ctl_h[C_BUSY_MAX];
       stats->diag[7] = shape.waves;
+      stats->diag[8] = ctl_h[C_ADJ_TESTS];
+      stats->diag[9] = ctl_h[C_ADJ_FREE];
+      stats->diag[10] = ctl_h[C_FAR_TESTS];
+      stats->diag[11] = ctl_h[C_FAR_JOINS];
     }
     if (npass >= kMaxExpandPasses) return DCLUST_E_CAPACITY;
     return rc;

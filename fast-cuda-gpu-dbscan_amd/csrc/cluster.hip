@@ -402,132 +402,217 @@ __device__ __forceinline__ float cell_gap(int o, int32_t f) {
   return (float)g;
 }
 
-// Light cell, one lane per core point p: for every light stencil cell that
-// can hold a point within eps of p (cell-box distance, the same fixed-point
-// arithmetic as the distance, so never above it) and is not yet known
-// adjacent, test its core points until one is joined to p.
+// Distance (fixed-point units) from a point at fraction f of its cell to
+// the far side of the cell o cells away along one axis.
+__device__ __forceinline__ float cell_far(int o, int32_t f) {
+  constexpr int32_t W = 1 << kFixBits;
+  const int32_t g = o > 0 ? (o + 1) * W - f : (o < 0 ? f - o * W : (f > W - f ? f : W - f));
+  return (float)g;
+}
+
+// Near and far stencil cells.  A stencil cell is "near" when it is at most
+// one cell away on every axis: the cells of lines 0..NN-1 (the lines whose
+// offsets are at most 1) with a last-axis offset of at most 1.  The other
+// stencil cells are "far".  Adjacency over near cells is computed for every
+// core cell and drives the seed-list expansion; a far pair is decided only
+// when its two cells are still in different components after the
+// expansion and merge (k_far), which is rare: in the interior of a cluster
+// the near cells already connect everything.
+template <int D>
+struct Near {
+  static constexpr int lines = D == 3 ? 9 : 3;
+};
+
+// Is some core point of A (points [qs, qe)) joined to some core point of B
+// (points [b0, b1), holding at least one core point)?  off = offset of B's
+// cell from A's.  A's core points are tried in turn: skipped when B's cell
+// box lies beyond eps of the point, accepted without a test when B's whole
+// box lies inside the guard band around it (every point of B is then
+// joined to it), else tested against B's core points.  The box distances
+// use the same fixed-point arithmetic as the distance, so they bound it.
 template <int D, bool F32>
-__device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, int32_t c,
-                         unsigned long long& m0, unsigned long long& m1, Tally& t) {
-  constexpr int NR = D == 3 ? 25 : 5;
-  constexpr int B = Batch<F32>::value;
+__device__ bool cells_joined(const Dev& d, const Par& P, int32_t qs, int32_t qe, bool allc,
+                             int32_t b0, int32_t b1, int off0, int off1, int offl, Tally& t,
+                             long long& freeb) {
   constexpr uint32_t kFrac = (1u << kFixBits) - 1u;
-  uint32_t pg[D];
-  double px[D];
-  load_g<D>(d, p, pg);
-  if (!F32) load_x<D>(d, p, px);
-  int32_t fr[D];
+  const int off[3] = {off0, D == 3 ? off1 : offl, offl};
+  for (int32_t a = qs; a < qe; ++a) {
+    if (!allc && !d.core[a]) continue;
+    uint32_t ag[D];
+    load_g<D>(d, a, ag);
+    float g2 = 0.f, h2 = 0.f;
 #pragma unroll
-  for (int k = 0; k < D; ++k) fr[k] = (int32_t)(pg[k] & kFrac);
-  const int64_t keyA = (int64_t)d.cell_key[c];
-  const int64_t nkl = I.nk[D - 1];
-  const int64_t nk1 = I.nk[1];
-  const int32_t* rows = d.rows + (int64_t)c * kMaxRows;
-  for (int r = 0; r < NR; ++r) {
-    const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
-    const int o1 = D == 3 ? kRowOff3[r][1] : 0;
-    float g2 = 0.f;
-    {
-      const float a = cell_gap(o0, fr[0]);
-      g2 = a * a;
-      if (D == 3) {
-        const float b = cell_gap(o1, fr[1]);
-        g2 = __builtin_fmaf(b, b, g2);
-      }
+    for (int x = 0; x < D; ++x) {
+      const int32_t f = (int32_t)(ag[x] & kFrac);
+      const float e = cell_gap(off[x], f), fe = cell_far(off[x], f);
+      g2 = __builtin_fmaf(e, e, g2);
+      h2 = __builtin_fmaf(fe, fe, h2);
     }
-    if (g2 > P.d2_out) continue;  // the whole line is beyond eps
-    const int32_t pk = rows[r];
-    const int32_t len = pk >> kRowCntShift;
-    if (len == 0) continue;
-    const int32_t st = pk & kRowCellMask;
-    int32_t bnd[6];
-    int64_t key[5];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) bnd[k] = k <= len ? d.cell_start[st + k] : 0;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) key[k] = k < len ? (int64_t)d.cell_key[st + k] : 0;
-    const int64_t line_delta = D == 3 ? ((int64_t)o0 * nk1 + o1) * nkl : (int64_t)o0 * nkl;
-    int32_t aa[5], ll[5];
-    const int base = 5 * r;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      const int32_t n = k < len ? bnd[k + 1] - bnd[k] : 0;
-      const int ol = (int)(key[k] - keyA - line_delta);  // offset along the last axis
-      const float e = cell_gap(ol, fr[D - 1]);
-      const bool near = !(__builtin_fmaf(e, e, g2) > P.d2_out);
-      const bool use =
-          n > 0 && n <= kHeavy && st + k != c && near && !mask_has(m0, m1, base + k);
-      aa[k] = bnd[k];
-      ll[k] = use ? n : 0;
+    if (g2 > P.d2_out) continue;
+    if (h2 < P.d2_in) {
+      ++freeb;
+      return true;
     }
-    Stream5 s;
-    s.set(aa, ll);
-    for (int32_t t0 = 0; t0 < s.tot; t0 += B) {
-      int32_t qi[B];
-      int kk[B];
-      bool cf[B];
-      uint32_t cg[B][D];
-      double cx[B][D];
+    double ax[D];
 #pragma unroll
-      for (int j = 0; j < B; ++j) {
-        int jr = 0;
-        const bool in = t0 + j < s.tot;
-        qi[j] = in ? s.at(t0 + j, jr) : p;
-        kk[j] = jr;
-        const bool need = in && !mask_has(m0, m1, base + jr);
-        cf[j] = need && d.core[qi[j]];
+    for (int x = 0; x < D; ++x) ax[x] = F32 ? 0.0 : d.X[x][a];
+    for (int32_t q0 = b0; q0 < b1; q0 += 4) {
+      bool cf[4];
+      uint32_t qg[4][D];
+      double qx[4][D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-          if (F32)
-            cg[j][k] = need ? d.G[k][qi[j]] : 0u;
-          else
-            cx[j][k] = need ? d.X[k][qi[j]] : 0.0;
+      for (int j = 0; j < 4; ++j) {
+        const bool in = q0 + j < b1;
+        const int32_t q = in ? q0 + j : b0;
+        cf[j] = in && d.core[q];
+#pragma unroll
+        for (int x = 0; x < D; ++x) {
+          qg[j][x] = (F32 && cf[j]) ? d.G[x][q] : 0u;
+          qx[j][x] = (!F32 && cf[j]) ? d.X[x][q] : 0.0;
         }
       }
 #pragma unroll
-      for (int j = 0; j < B; ++j) {
-        if (cf[j] && !mask_has(m0, m1, base + kk[j]) &&
-            edge_pair<D, F32>(d, P, p, qi[j], pg, cg[j], px, cx[j], t))
-          mask_set(m0, m1, base + kk[j]);
+      for (int j = 0; j < 4; ++j) {
+        if (cf[j]) {
+          ++t.eval;
+          if (edge_pair<D, F32>(d, P, a, q0 + j, ag, qg[j], ax, qx[j], t)) return true;
+        }
       }
     }
   }
+  return false;
+}
+
+// Offset along the last axis of cell B (in stencil line r of cell A).
+template <int D>
+__device__ __forceinline__ int cell_offset_last(const Dev& d, const Inst& I, int32_t A, int32_t B,
+                                                int r) {
+  const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
+  const int o1 = D == 3 ? kRowOff3[r][1] : 0;
+  const int64_t nkl = I.nk[D - 1];
+  const int64_t line_delta = D == 3 ? ((int64_t)o0 * I.nk[1] + o1) * nkl : (int64_t)o0 * nkl;
+  return (int)((int64_t)d.cell_key[B] - (int64_t)d.cell_key[A] - line_delta);
+}
+
+// The cells of stencil line r of cell A: first cell, count, point bounds,
+// core counts and offsets along the last axis.
+struct LineCells {
+  int32_t st, len;
+  int32_t b0[5], b1[5];  // point range of each cell
+  int32_t nco[5];
+  int ol[5];
+};
+
+template <int D>
+__device__ __forceinline__ void line_cells(const Dev& d, const Inst& I, int32_t A, int r,
+                                           LineCells& L) {
+  const int32_t pk = d.rows[(int64_t)A * kMaxRows + r];
+  L.len = pk >> kRowCntShift;
+  L.st = pk & kRowCellMask;
+  int64_t key[5];
+  int32_t bnd[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) bnd[k] = k <= L.len ? d.cell_start[L.st + k] : 0;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    L.b0[k] = bnd[k];
+    L.b1[k] = bnd[k + 1];
+    key[k] = k < L.len ? (int64_t)d.cell_key[L.st + k] : 0;
+    L.nco[k] = k < L.len ? d.cell_ncore[L.st + k] : 0;
+  }
+  const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
+  const int o1 = D == 3 ? kRowOff3[r][1] : 0;
+  const int64_t nkl = I.nk[D - 1];
+  const int64_t line_delta = D == 3 ? ((int64_t)o0 * I.nk[1] + o1) * nkl : (int64_t)o0 * nkl;
+  const int64_t keyA = (int64_t)d.cell_key[A];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) L.ol[k] = (int)(key[k] - keyA - line_delta);
+}
+
+// Light core cell A, near line r (one lane; 16 lanes per cell): bits of the
+// adjacent light near cells of the line.  Heavy neighbours are left to the
+// heavy-cell pass, which sets the mirrored bit.
+template <int D, bool F32>
+__device__ unsigned adj_near_line(const Dev& d, const Inst& I, const Par& P, int32_t A, int r,
+                                  int32_t qs, int32_t qe, bool allc, Tally& t,
+                                  long long& freeb) {
+  LineCells L;
+  line_cells<D>(d, I, A, r, L);
+  const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
+  const int o1 = D == 3 ? kRowOff3[r][1] : 0;
+  unsigned todo = 0u;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const int ol = L.ol[k];
+    if (k < L.len && L.st + k != A && ol >= -1 && ol <= 1 && L.nco[k] > 0 &&
+        L.b1[k] - L.b0[k] <= kHeavy)
+      todo |= 1u << k;
+  }
+  // one copy of the decision: the chosen cell's data is read again (cache
+  // hits) instead of being selected from register arrays
+  unsigned bits = 0u;
+  while (todo) {
+    const int k = __ffs(todo) - 1;
+    todo &= todo - 1;
+    const int32_t B = L.st + k;
+    const int ol = cell_offset_last<D>(d, I, A, B, r);
+    if (cells_joined<D, F32>(d, P, qs, qe, allc, d.cell_start[B], d.cell_start[B + 1], o0, o1,
+                             ol, t, freeb))
+      bits |= 1u << k;
+  }
+  return bits;
 }
 
 template <bool F32>
 __global__ __launch_bounds__(256) void k_adj(Dev d) {
-  const int64_t pl = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  bool act = pl < d.n;
-  const int32_t p = act ? (int32_t)pl : 0;
-  int32_t c = 0;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t A64 = tid >> 4;
+  const int r = (int)(tid & 15);
+  const int64_t C = d.ctl[C_NCELL];
+  const int32_t A = A64 < C ? (int32_t)A64 : 0;
+  int32_t qs = 0, qe = 0, ncA = 0;
   int dim = 0;
   const Inst* I = d.inst;
-  if (act) {
-    c = d.cell_of[p];
-    act = d.core[p] && d.cell_start[c + 1] - d.cell_start[c] <= kHeavy;
-    I = d.inst + d.cell_inst[c];
+  if (A64 < C) {
+    qs = d.cell_start[A];
+    qe = d.cell_start[A + 1];
+    ncA = d.cell_ncore[A];
+    I = d.inst + d.cell_inst[A];
     dim = I->dim;
   }
+  const bool light = A64 < C && ncA > 0 && qe - qs <= kHeavy;
+  const bool act = light && r < (dim == 3 ? Near<3>::lines : Near<2>::lines);
   Tally t;
-  unsigned long long m0 = 0, m1 = 0;
+  long long freeb = 0;
+  unsigned bits = 0u;
   const Par P(*I);
-  DCLUST_BY_DIM(act, dim, (adj_lane<3, F32>(d, *I, P, p, c, m0, m1, t)),
-                (adj_lane<2, F32>(d, *I, P, p, c, m0, m1, t)));
-  if (m0) atomicOr(&d.amask[2 * (int64_t)c], m0);
-  if (m1) atomicOr(&d.amask[2 * (int64_t)c + 1], m1);
+  const bool allc = ncA == qe - qs;
+  DCLUST_BY_DIM(act, dim, (bits = adj_near_line<3, F32>(d, *I, P, A, r, qs, qe, allc, t, freeb)),
+                (bits = adj_near_line<2, F32>(d, *I, P, A, r, qs, qe, allc, t, freeb)));
+  // OR the lines' bits (slot 5r+k, r < 9) over the 16 lanes of the cell
+  unsigned long long m0 = (unsigned long long)bits << (5 * r);
+#pragma unroll
+  for (int o = 1; o < 16; o <<= 1) m0 |= __shfl_xor(m0, o, 64);
+  // light core cells own their mask here (zeroed by k_core; the heavy pass
+  // only ORs mirrored bits into it afterwards)
+  if (light && r == 0) d.amask[2 * (int64_t)A] = m0;
   wave_add(&d.ctl[C_BAND], t.band);
   wave_add(&d.ctl[C_ASYM], t.asym);
+  wave_add(&d.ctl[C_ADJ_TESTS], t.eval);
+  wave_add(&d.ctl[C_ADJ_FREE], freeb);
 }
 
-// Heavy cell A, one wavefront: its core points are the queries (the first
-// kQCache cached in LDS).  Each stencil cell with core points is decided in
-// turn: its core points within eps of the queries' bounding box are the
-// candidates (one per lane), and the queries near the candidates' bounding
-// box are tried until some lane finds a joined pair.
+// Heavy cell A (more than kHeavy points), one wavefront, near cells only:
+// its core points are the queries (the first kQCache cached in LDS).  Each
+// near cell with core points is decided in turn: its core points within
+// eps of the queries' bounding box are the candidates (one per lane), and
+// the queries near the candidates' bounding box are tried until some lane
+// finds a joined pair.  A joined light neighbour also gets the mirrored bit
+// (A lies in its mirrored stencil line).
 template <int D, bool F32>
-__device__ void adj_heavy(const Dev& d, const Par& P, int32_t A, int32_t* s_qi, uint32_t* s_qg,
-                          double* s_qx, Tally& t) {
-  constexpr int NR = D == 3 ? 25 : 5;
+__device__ void adj_heavy(const Dev& d, const Inst& I, const Par& P, int32_t A, int32_t* s_qi,
+                          uint32_t* s_qg, double* s_qx, Tally& t) {
+  constexpr int NN = Near<D>::lines;
   const int lane = lane_id();
   const int32_t qs = d.cell_start[A], qe = d.cell_start[A + 1];
   const int32_t ncA = d.cell_ncore[A];
@@ -570,14 +655,21 @@ __device__ void adj_heavy(const Dev& d, const Par& P, int32_t A, int32_t* s_qi, 
   wave_sync();
   const bool cached = nq <= kQCache;
   const int NQ = cached ? nq : (qe - qs);
-  const int32_t pkl = lane < NR ? d.rows[(int64_t)A * kMaxRows + lane] : 0;
-  unsigned long long m0 = 0, m1 = 0;
-  for (int r = 0; r < NR; ++r) {
+  const int32_t pkl = lane < NN ? d.rows[(int64_t)A * kMaxRows + lane] : 0;
+  const int64_t keyA = (int64_t)d.cell_key[A];
+  const int64_t nkl = I.nk[D - 1];
+  unsigned long long m0 = 0;
+  for (int r = 0; r < NN; ++r) {
     const int32_t pk = rl_i(pkl, r);
     const int32_t len = pk >> kRowCntShift, st = pk & kRowCellMask;
+    const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
+    const int o1 = D == 3 ? kRowOff3[r][1] : 0;
+    const int64_t line_delta = D == 3 ? ((int64_t)o0 * I.nk[1] + o1) * nkl : (int64_t)o0 * nkl;
     for (int k = 0; k < len; ++k) {
       const int32_t Bc = st + k;
       if (Bc == A) continue;
+      const int ol = (int)((int64_t)d.cell_key[Bc] - keyA - line_delta);
+      if (ol < -1 || ol > 1) continue;  // a far cell
       const int32_t b0 = d.cell_start[Bc], b1 = d.cell_start[Bc + 1];
       if (d.cell_ncore[Bc] == 0) continue;
       bool edge = false;
@@ -624,22 +716,17 @@ __device__ void adj_heavy(const Dev& d, const Par& P, int32_t A, int32_t* s_qi, 
         }
       }
       if (edge) {
-        mask_set(m0, m1, 5 * r + k);
+        m0 |= 1ull << (5 * r + k);
         if (b1 - b0 <= kHeavy) {
-          // light neighbour: set the bit of A in its mask (A is in its
-          // mirrored stencil line)
           const int rm = D == 3 ? kRowMirror3[r] : kRowMirror2[r];
           const int32_t pm = d.rows[(int64_t)Bc * kMaxRows + rm];
           const int bm = 5 * rm + (A - (pm & kRowCellMask));
-          if (lane == 0) atomicOr(&d.amask[2 * (int64_t)Bc + (bm >> 6)], 1ull << (bm & 63));
+          if (lane == 0) atomicOr(&d.amask[2 * (int64_t)Bc], 1ull << bm);
         }
       }
     }
   }
-  if (lane == 0) {
-    d.amask[2 * (int64_t)A] = m0;
-    d.amask[2 * (int64_t)A + 1] = m1;
-  }
+  if (lane == 0) d.amask[2 * (int64_t)A] = m0;
 }
 
 template <bool F32>
@@ -654,13 +741,101 @@ __global__ __launch_bounds__(64) void k_adj_heavy(Dev d) {
     const Inst& I = d.inst[d.cell_inst[A]];
     const Par P(I);
     if (I.dim == 3)
-      adj_heavy<3, F32>(d, P, A, s_qi, s_qg, s_qx, t);
+      adj_heavy<3, F32>(d, I, P, A, s_qi, s_qg, s_qx, t);
     else
-      adj_heavy<2, F32>(d, P, A, s_qi, s_qg, s_qx, t);
+      adj_heavy<2, F32>(d, I, P, A, s_qi, s_qg, s_qx, t);
     wave_sync();  // the query cache is reused
   }
   wave_add(&d.ctl[C_BAND], t.band);
   wave_add(&d.ctl[C_ASYM], t.asym);
+}
+
+// ------------------------------------------------------------ far pairs --
+// Component of every core cell after the expansion and merge: the root of
+// its chain.
+__global__ __launch_bounds__(256) void k_cell_comp(Dev d) {
+  const int64_t C = d.ctl[C_NCELL];
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < C;
+       c += (int64_t)gridDim.x * blockDim.x)
+    d.cell_comp[c] = d.cell_ncore[c] > 0 ? d.parent[d.cell_label[c]] : -1;
+}
+
+// Core cell A, stencil line r (one lane; 32 lanes per cell): its far cells
+// with core points in another component are decided, each unordered pair by
+// its lower cell; a joined pair unions the two components (lock-free
+// union-find on the chain forest).
+template <int D, bool F32>
+__device__ long long far_line(const Dev& d, const Inst& I, const Par& P, int32_t A, int r,
+                              int32_t qs, int32_t qe, bool allc, int32_t compA, Tally& t,
+                              long long& freeb) {
+  LineCells L;
+  line_cells<D>(d, I, A, r, L);
+  const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
+  const int o1 = D == 3 ? kRowOff3[r][1] : 0;
+  const bool far_l = o0 == 2 || o0 == -2 || o1 == 2 || o1 == -2;
+  int32_t comp[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) comp[k] = k < L.len ? d.cell_comp[L.st + k] : compA;
+  unsigned todo = 0u;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const int ol = L.ol[k];
+    const bool far = far_l || ol == 2 || ol == -2;
+    if (k < L.len && far && L.st + k > A && L.nco[k] > 0 && comp[k] != compA) todo |= 1u << k;
+  }
+  long long joins = 0;
+  while (todo) {
+    const int k = __ffs(todo) - 1;
+    todo &= todo - 1;
+    const int32_t B = L.st + k;
+    const int ol = cell_offset_last<D>(d, I, A, B, r);
+    if (cells_joined<D, F32>(d, P, qs, qe, allc, d.cell_start[B], d.cell_start[B + 1], o0, o1,
+                             ol, t, freeb)) {
+      uf_union(d.parent, compA, d.cell_comp[B]);
+      ++joins;
+    }
+  }
+  return joins;
+}
+
+template <bool F32>
+__global__ __launch_bounds__(256) void k_far(Dev d) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t A64 = tid >> 5;
+  const int r = (int)(tid & 31);
+  const int64_t C = d.ctl[C_NCELL];
+  const int32_t A = A64 < C ? (int32_t)A64 : 0;
+  int32_t qs = 0, qe = 0, ncA = 0, compA = -1;
+  int dim = 0;
+  const Inst* I = d.inst;
+  if (A64 < C) {
+    qs = d.cell_start[A];
+    qe = d.cell_start[A + 1];
+    ncA = d.cell_ncore[A];
+    compA = d.cell_comp[A];
+    I = d.inst + d.cell_inst[A];
+    dim = I->dim;
+  }
+  const bool act = A64 < C && ncA > 0 && r < (dim == 3 ? 25 : 5);
+  Tally t;
+  long long freeb = 0, joins = 0;
+  const Par P(*I);
+  const bool allc = ncA == qe - qs;
+  DCLUST_BY_DIM(act, dim,
+                (joins = far_line<3, F32>(d, *I, P, A, r, qs, qe, allc, compA, t, freeb)),
+                (joins = far_line<2, F32>(d, *I, P, A, r, qs, qe, allc, compA, t, freeb)));
+  wave_add(&d.ctl[C_BAND], t.band);
+  wave_add(&d.ctl[C_ASYM], t.asym);
+  wave_add(&d.ctl[C_FAR_TESTS], t.eval);
+  wave_add(&d.ctl[C_FAR_JOINS], joins);
+}
+
+// Roots of every chain after the far-pair unions.
+__global__ __launch_bounds__(256) void k_flatten_all(Dev d) {
+  const long long nc = d.ctl[C_NCHAIN];
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc;
+       c += (long long)gridDim.x * blockDim.x)
+    d.parent[c] = uf_find(d.parent, (int32_t)c);
 }
 
 // ---------------------------------------------------------------- expand --
@@ -1195,6 +1370,13 @@ static int point_grid(int64_t n) {
   return (int)(b < 8 ? 8 : b);
 }
 
+// 32 lanes per cell, for up to n cells (the cell count is known on the
+// device only; lanes past it exit at once)
+static int cell_line_grid(int64_t n, int lanes) {
+  int64_t b = (n * lanes + 255) / 256;
+  return (int)(b < 1 ? 1 : b);
+}
+
 void launch_core(const Dev& d, bool f32, hipStream_t s) {
   if (f32)
     hipLaunchKernelGGL(k_core<true>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
@@ -1204,10 +1386,10 @@ void launch_core(const Dev& d, bool f32, hipStream_t s) {
 
 void launch_adj(const Dev& d, bool f32, int heavy_grid, hipStream_t s) {
   if (f32) {
-    hipLaunchKernelGGL(k_adj<true>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_adj<true>, dim3(cell_line_grid(d.n, 16)), dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_adj_heavy<true>, dim3(heavy_grid), dim3(64), 0, s, d);
   } else {
-    hipLaunchKernelGGL(k_adj<false>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_adj<false>, dim3(cell_line_grid(d.n, 16)), dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_adj_heavy<false>, dim3(heavy_grid), dim3(64), 0, s, d);
   }
 }
@@ -1242,6 +1424,15 @@ void launch_merge(const Dev& d, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_merge_lds, dim3(1), dim3(1024), 0, s, d);
   hipLaunchKernelGGL(k_union_pairs, dim3(grid), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_flatten, dim3(grid), dim3(256), 0, s, d);
+}
+
+void launch_far(const Dev& d, bool f32, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_cell_comp, dim3(grid), dim3(256), 0, s, d);
+  if (f32)
+    hipLaunchKernelGGL(k_far<true>, dim3(cell_line_grid(d.n, 32)), dim3(256), 0, s, d);
+  else
+    hipLaunchKernelGGL(k_far<false>, dim3(cell_line_grid(d.n, 32)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_flatten_all, dim3(grid), dim3(256), 0, s, d);
 }
 
 void launch_border(const Dev& d, bool f32, hipStream_t s) {

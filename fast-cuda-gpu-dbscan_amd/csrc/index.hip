@@ -536,6 +536,30 @@ __device__ __forceinline__ int32_t inst_of_key(const Dev& d, uint64_t key) {
   return lo;
 }
 
+// Integer cell coordinates of a key, decoded once per cell: 21 bits per
+// axis in 3-D (at most 2^21 cells per axis), 32 bits per axis in 2-D.
+__device__ __forceinline__ uint64_t pack_cell(const Inst& I, uint64_t key) {
+  int64_t lin = (int64_t)(key - (uint64_t)I.key_base);
+  uint64_t kc[3] = {0, 0, 0};
+  for (int k = I.dim - 1; k >= 0; --k) {
+    kc[k] = (uint64_t)(lin % I.nk[k]);
+    lin /= I.nk[k];
+  }
+  return I.dim == 3 ? (kc[0] | (kc[1] << 21) | (kc[2] << 42)) : (kc[0] | (kc[1] << 32));
+}
+
+__device__ __forceinline__ void unpack_cell(int dim, uint64_t pc, int32_t* kc) {
+  if (dim == 3) {
+    kc[0] = (int32_t)(pc & 0x1fffff);
+    kc[1] = (int32_t)((pc >> 21) & 0x1fffff);
+    kc[2] = (int32_t)(pc >> 42);
+  } else {
+    kc[0] = (int32_t)(pc & 0xffffffffu);
+    kc[1] = (int32_t)(pc >> 32);
+    kc[2] = 0;
+  }
+}
+
 __global__ __launch_bounds__(CS_T) void k_csr_write(Dev d) {
   const uint64_t* key = sorted_keys(d);
   const int64_t t0 = (int64_t)blockIdx.x * CS_TILE;
@@ -558,9 +582,11 @@ __global__ __launch_bounds__(CS_T) void k_csr_write(Dev d) {
     if (inr) {
       d.cell_of[i] = (int32_t)cid;
       if (f) {
+        const int32_t inst = inst_of_key(d, k);
         d.cell_start[cid] = (int32_t)i;
         d.cell_key[cid] = k;
-        d.cell_inst[cid] = inst_of_key(d, k);
+        d.cell_inst[cid] = inst;
+        d.cell_xyz[cid] = pack_cell(d.inst[inst], k);
       }
       if (i == d.n - 1) {
         d.cell_start[cid + 1] = (int32_t)d.n;
@@ -615,8 +641,9 @@ __device__ __forceinline__ int32_t lower_bound_key_in(const uint64_t* a, int32_t
 // line of a cell: all coordinates but the last; global line-table index
 __device__ __forceinline__ int64_t cell_line(const Dev& d, int32_t c) {
   const Inst& I = d.inst[d.cell_inst[c]];
-  const int64_t lin = (int64_t)(d.cell_key[c] - (uint64_t)I.key_base);
-  return I.line_base + lin / I.nk[I.dim - 1];
+  int32_t kc[3];
+  unpack_cell(I.dim, d.cell_xyz[c], kc);
+  return I.line_base + (I.dim == 3 ? (int64_t)kc[0] * I.nk[1] + kc[1] : (int64_t)kc[0]);
 }
 
 // Line table: for every possible line, the range of its non-empty cells in
@@ -658,12 +685,8 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
     const int nrow = dim == 3 ? 25 : 5;
     int32_t packed = 0;
     if (r < nrow) {
-      int64_t lin = (int64_t)(d.cell_key[c] - (uint64_t)I.key_base);
-      int32_t kc[3] = {0, 0, 0};
-      for (int k = dim - 1; k >= 0; --k) {
-        kc[k] = (int32_t)(lin % I.nk[k]);
-        lin /= I.nk[k];
-      }
+      int32_t kc[3];
+      unpack_cell(dim, d.cell_xyz[c], kc);
       const int last = dim - 1;
       int o0, o1 = 0;
       if (dim == 3) {
@@ -689,9 +712,29 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
           le = lr.y;
         }
         if (ls >= 0) {
-          const int32_t a = lower_bound_key_in(d.cell_key, ls, le, klo);
-          int32_t b = a;
-          while (b < le && d.cell_key[b] <= khi) ++b;
+          // cells of the line with keys in [klo, khi] (at most 5): a short
+          // line is read in one go, a long one searched
+          int32_t a, b = 0;
+          if (le - ls <= 8) {
+            uint64_t kk[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) kk[j] = ls + j < le ? d.cell_key[ls + j] : ~0ull;
+            a = ls;
+            b = ls;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              a += kk[j] < klo;
+              b += kk[j] <= khi;
+            }
+          } else {
+            a = lower_bound_key_in(d.cell_key, ls, le, klo);
+            uint64_t kk[5];
+#pragma unroll
+            for (int j = 0; j < 5; ++j) kk[j] = a + j < le ? d.cell_key[a + j] : ~0ull;
+            b = a;
+#pragma unroll
+            for (int j = 0; j < 5; ++j) b += kk[j] <= khi;
+          }
           packed = (b > a) ? (a | ((b - a) << kRowCntShift)) : 0;
           nbrc += b - a;
           cand += (long long)(d.cell_start[c + 1] - d.cell_start[c]) *

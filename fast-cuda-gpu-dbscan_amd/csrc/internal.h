@@ -126,6 +126,10 @@ enum Ctl : int {
   C_NHEAVY,     // heavy cells (listed in Dev::heavy)
   C_EDGES,      // adjacency bits of the expanded cells
   C_RECOVF,     // non-core points whose neighbour record overflowed
+  C_ADJ_TESTS,  // pair tests of the adjacency lanes
+  C_ADJ_FREE,   // adjacency bits set without a test (cell inside the eps ball)
+  C_FAR_TESTS,  // pair tests of the far-pair pass
+  C_FAR_JOINS,  // far pairs that joined two components
   C_NCTL = 36
 };
 
@@ -156,10 +160,12 @@ struct Dev {
   // CSR
   int32_t* cell_start;  // [N+1]
   uint64_t* cell_key;   // [N]
+  uint64_t* cell_xyz;   // [N] packed integer cell coordinates
   int32_t* cell_inst;   // [N]
   int32_t* cell_ncore;  // [N] core points per cell
   int32_t* cell_label;  // [N] chain owning the cell's core points
-  unsigned long long* amask;  // [N][2] adjacency: bit 5r+k <-> k-th cell of line r
+  unsigned long long* amask;  // [N][2] near adjacency: bit 5r+k <-> k-th cell of line r
+  int32_t* cell_comp;   // [N] component (root chain) of each core cell, -1 otherwise
   int32_t* heavy;       // heavy cells (more than kHeavy points)
   int32_t* rows;        // [N][kMaxRows] packed neighbour lines (first cell, count)
   int32_t* cell_of;     // [N] per sorted point

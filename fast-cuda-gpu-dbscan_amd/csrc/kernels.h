@@ -30,6 +30,8 @@ void launch_seg_init(const Dev& d, int64_t waves, hipStream_t s);
 void launch_expand(const Dev& d, int pass, int grid, int seeds, int threads, int budget,
                    hipStream_t s);
 void launch_merge(const Dev& d, int grid, hipStream_t s);
+// far stencil pairs across components, then final roots
+void launch_far(const Dev& d, bool f32, int grid, hipStream_t s);
 void launch_border(const Dev& d, bool f32, hipStream_t s);
 void launch_relabel(const Dev& d, int64_t n, hipStream_t s);
 

@@ -51,7 +51,7 @@ class Params(C.Structure):
     ]
 
 
-_STAT_F64 = ["t_total_ms", "t_index_ms", "t_core_ms", "t_adj_ms", "t_expand_ms", "t_merge_ms",
+_STAT_F64 = ["t_total_ms", "t_index_ms", "t_core_ms", "t_adj_ms", "t_expand_ms", "t_merge_ms", "t_far_ms",
              "t_border_ms", "t_relabel_ms"]
 _STAT_I64 = ["n_points", "n_cells", "n_core", "n_noise", "n_clusters", "n_chains",
              "n_collisions", "n_overflow", "n_passes", "cand_total", "nbr_cells_total",
@@ -62,7 +62,7 @@ _STAT_I32 = ["hip_error", "key_bits"]
 
 class Stats(C.Structure):
     _fields_ = ([(k, C.c_double) for k in _STAT_F64] + [(k, C.c_int64) for k in _STAT_I64] +
-                [(k, C.c_int32) for k in _STAT_I32] + [("diag", C.c_int64 * 8)])
+                [(k, C.c_int32) for k in _STAT_I32] + [("diag", C.c_int64 * 12)])
 
     def as_dict(self):
         out = {k: getattr(self, k) for k, _ in self._fields_ if k != "diag"}

@@ -80,9 +80,10 @@ typedef struct dclust_stats {
   double t_total_ms;   /* first kernel to labels resident on the device   */
   double t_index_ms;   /* bbox + cell keys + radix sort + CSR + rows        */
   double t_core_ms;    /* neighbour-count (core flag) kernel                */
-  double t_adj_ms;     /* cell adjacency (core-core pairs within eps)       */
+  double t_adj_ms;     /* near-cell adjacency (core-core pairs within eps)  */
   double t_expand_ms;  /* seed-list expansion passes                        */
   double t_merge_ms;   /* union-find merge of collision pairs               */
+  double t_far_ms;     /* far cell pairs across components (+ unions)       */
   double t_border_ms;  /* border assignment                                 */
   double t_relabel_ms; /* canonical relabel + output                        */
   int64_t n_points;
@@ -109,8 +110,11 @@ typedef struct dclust_stats {
    * distance tests, [2] heavy cells (adjacency by one wavefront per cell),
    * [3] adjacent cell pairs (directed), [4] time of expansion pass 0 (us),
    * [5] sum over expansion wavefronts of their lifetime (us), [6] longest
-   * lifetime (us), [7] expansion wavefronts */
-  int64_t diag[8];
+   * lifetime (us), [7] expansion wavefronts, [8] pair tests of the
+   * near-adjacency kernel, [9] adjacency bits set without a test, [10]
+   * pair tests of the far-pair pass, [11] far pairs that joined two
+   * components */
+  int64_t diag[12];
 } dclust_stats;
 
 typedef struct dclust_handle dclust_handle;

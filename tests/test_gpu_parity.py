@@ -126,3 +126,36 @@ def test_batched_matches_oracle(handle):
     assert not np.any(errs)
     for i, (a, b) in enumerate(zip(labs, refs)):
         assert np.array_equal(a, b), f"instance {i}"
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("dim", [2, 3])
+def test_far_links_only(handle, prec, dim):
+    # tight groups 0.97 eps apart along the first axis: consecutive groups
+    # sit two cells apart, so only far cell pairs join them
+    rng = np.random.default_rng(11 + dim)
+    eps, mp = 0.1, 4
+    groups = []
+    for g in range(40):
+        c = np.zeros(dim)
+        c[0] = g * 0.97 * eps
+        c[1:] = rng.uniform(0, 1, dim - 1) * 1e-3
+        groups.append(c + rng.normal(0, 0.002 * eps, (6, dim)))
+    x = np.concatenate(groups + [rng.uniform(-1, 5, (300, dim))])
+    st = _check(handle, x, eps, mp, prec)
+    assert st["diag"][11] > 0  # components joined by far pairs
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("minpts", [5, 60])
+def test_heavy_cells(handle, prec, dim, minpts):
+    # dense blobs: cells with far more than 32 (and 256) points, with and
+    # without all-core cells
+    rng = np.random.default_rng(21 + dim + minpts)
+    eps = 0.05
+    parts = [rng.normal(0, 0.3 * eps, (30000, dim)), rng.normal(0.6, 1.5 * eps, (20000, dim)),
+             rng.uniform(-0.5, 1.5, (5000, dim))]
+    x = np.concatenate(parts)
+    st = _check(handle, x, eps, minpts, prec)
+    assert st["diag"][2] > 0  # heavy cells present
