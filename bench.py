@@ -87,9 +87,10 @@ class Hip:
 def b_nbr(st, dim, bytes_per_coord):
     """Algorithmic bytes of one neighbour-count launch: every point read once
     as a query and once per neighbouring cell tile (sum over cells of n_c
-    times the points of its 3^D neighbourhood), 8 bytes per neighbour-cell
-    range entry, 1 byte of core flag written per point, plus 2*D*8 bytes of
-    FP64 coordinates per guard-band recheck."""
+    times the points of its 5^D stencil of half-eps cells), 8 bytes per
+    stencil-cell range entry, 1 byte of core flag written per point, plus
+    2*D*8 bytes of FP64 coordinates per guard-band recheck.  The sums come
+    from one untimed run with DCLUST_NBR_STATS=1."""
     n = st["n_points"]
     return (bytes_per_coord * dim * (n + st["cand_total"]) + 8 * st["nbr_cells_total"] + n +
             16 * dim * st.get("band_rechecks_core", 0))
@@ -166,8 +167,14 @@ def main():
         hip.sync()
         return time.perf_counter() - t0, sts
 
+    # neighbourhood sizes for the algorithmic-byte count: one extra untimed run
+    os.environ["DCLUST_NBR_STATS"] = "1"
+    st_nbr = h.run_device(d_x, n, p, d_lab)
+    del os.environ["DCLUST_NBR_STATS"]
     dt, sts = timed(p, args.steps, args.warmup)
-    st = sts[-1]
+    st = dict(sts[-1])
+    st["cand_total"] = st_nbr["cand_total"]
+    st["nbr_cells_total"] = st_nbr["nbr_cells_total"]
     ms_step = dt / args.steps * 1e3
     value = n * args.steps / dt / 1e6
     t_core = float(np.mean([s["t_core_ms"] for s in sts]))

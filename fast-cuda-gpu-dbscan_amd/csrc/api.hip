@@ -84,7 +84,7 @@ enum Slot {
   S_DTOT, S_PASS, S_CSTART, S_CKEY, S_CINST, S_CM, S_ROWS, S_CELLOF, S_X0, S_X1, S_X2,
   S_G0, S_G1, S_G2, S_ORIG, S_CORE, S_LABEL, S_PARENT, S_COMPMIN, S_PAIRS, S_Q0, S_Q1,
   S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT, S_PAIRCNT, S_SEGPOS, S_LINES, S_AMASK,
-  S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP
+  S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP, S_ADJPTS, S_CORECELLS
 };
 
 // line table entries (8 bytes each) above which neighbour lines are found
@@ -223,6 +223,12 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
   d.xyz = d_xyz;
   d.pair_cap = shape.pair_cap;
   d.pair_per = shape.pair_per;
+  // neighbourhood sizes (stats.cand_total / nbr_cells_total) cost an extra
+  // dependent load per stencil line; they are summed only on request
+  {
+    const char* e = std::getenv("DCLUST_NBR_STATS");
+    d.nbr_stats = (e && e[0] == '1') ? 1 : 0;
+  }
   // neighbour records of non-core points: minpts slots per point (a
   // non-core point has at most minpts - 1 neighbours outside its own cell,
   // plus pairs whose box test is asymmetric)
@@ -248,6 +254,8 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     d.cell_key = h->get<uint64_t>(S_CKEY, ntot, &E);
     d.cell_xyz = h->get<uint64_t>(S_CXYZ, ntot, &E);
     d.cell_comp = h->get<int32_t>(S_CCOMP, ntot, &E);
+    d.adj_pts = h->get<int32_t>(S_ADJPTS, ntot, &E);
+    d.core_cells = h->get<int32_t>(S_CORECELLS, ntot, &E);
     d.cell_inst = h->get<int32_t>(S_CINST, ntot, &E);
     d.cell_ncore = h->get<int32_t>(S_CM, ntot, &E);
     d.rows = h->get<int32_t>(S_ROWS, (size_t)ntot * kMaxRows, &E);

@@ -248,6 +248,29 @@ __device__ __forceinline__ void mask_set(unsigned long long& m0, unsigned long l
     }                                              \
   } while (0)
 
+// Appends `value` of the threads with `take` to `list`, in thread order,
+// with one atomic on `counter` per workgroup (a counter shared by a whole
+// launch is the contended address).  Every thread of the workgroup calls it.
+__device__ __forceinline__ void block_append(bool take, int32_t value, int32_t* list,
+                                             long long* counter) {
+  __shared__ int s_cnt[16];
+  __shared__ long long s_base;
+  const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const unsigned long long m = ballot(take);
+  if (lane_id() == 0) s_cnt[wv] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < nw; ++w) tot += s_cnt[w];
+    s_base = tot ? (long long)atomicAdd((unsigned long long*)counter, (unsigned long long)tot) : 0;
+  }
+  __syncthreads();
+  long long base = s_base;
+  for (int w = 0; w < wv; ++w) base += s_cnt[w];
+  if (take) list[base + mbcnt(m)] = value;
+  __syncthreads();  // s_cnt / s_base are reused by the next call
+}
+
 // ------------------------------------------------------------------ core --
 // Neighbour count of point p with early exit at minpts.  Every candidate
 // pair with d^2 <= eps^2 (either box direction) is recorded while the count
@@ -355,7 +378,7 @@ __global__ __launch_bounds__(256) void k_core(Dev d) {
     c = d.cell_of[p];
     qs = d.cell_start[c];
     qe = d.cell_start[c + 1];
-    I = d.inst + d.cell_inst[c];
+    I = inst_of_cell(d, c);
     dim = I->dim;
   }
   Tally t;
@@ -365,6 +388,8 @@ __global__ __launch_bounds__(256) void k_core(Dev d) {
   DCLUST_BY_DIM(act, dim, (core_lane<3, F32>(d, P, p, c, qs, qe, R, t, isc)),
                 (core_lane<2, F32>(d, P, p, c, qs, qe, R, t, isc)));
   if (act) d.core[p] = isc;
+  // core points of light cells: the lanes of the near-adjacency pass
+  block_append(isc && qe - qs <= kHeavy, p, d.adj_pts, &d.ctl[C_NADJPTS]);
   // per-cell core counts: the lanes of one cell form a contiguous run
   const bool head = act && (p == qs || lane == 0);
   const unsigned long long mh = ballot(head), mc = ballot(isc);
@@ -494,108 +519,164 @@ __device__ __forceinline__ int cell_offset_last(const Dev& d, const Inst& I, int
   return (int)((int64_t)d.cell_key[B] - (int64_t)d.cell_key[A] - line_delta);
 }
 
-// The cells of stencil line r of cell A: first cell, count, point bounds,
-// core counts and offsets along the last axis.
-struct LineCells {
-  int32_t st, len;
-  int32_t b0[5], b1[5];  // point range of each cell
-  int32_t nco[5];
-  int ol[5];
-};
-
-template <int D>
-__device__ __forceinline__ void line_cells(const Dev& d, const Inst& I, int32_t A, int r,
-                                           LineCells& L) {
-  const int32_t pk = d.rows[(int64_t)A * kMaxRows + r];
-  L.len = pk >> kRowCntShift;
-  L.st = pk & kRowCellMask;
-  int64_t key[5];
-  int32_t bnd[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) bnd[k] = k <= L.len ? d.cell_start[L.st + k] : 0;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    L.b0[k] = bnd[k];
-    L.b1[k] = bnd[k + 1];
-    key[k] = k < L.len ? (int64_t)d.cell_key[L.st + k] : 0;
-    L.nco[k] = k < L.len ? d.cell_ncore[L.st + k] : 0;
-  }
-  const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
-  const int o1 = D == 3 ? kRowOff3[r][1] : 0;
-  const int64_t nkl = I.nk[D - 1];
-  const int64_t line_delta = D == 3 ? ((int64_t)o0 * I.nk[1] + o1) * nkl : (int64_t)o0 * nkl;
-  const int64_t keyA = (int64_t)d.cell_key[A];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) L.ol[k] = (int)(key[k] - keyA - line_delta);
-}
-
-// Light core cell A, near line r (one lane; 16 lanes per cell): bits of the
-// adjacent light near cells of the line.  Heavy neighbours are left to the
-// heavy-cell pass, which sets the mirrored bit.
+// Near adjacency of light cells: one lane per core point p of a light cell
+// (compacted list written by k_core).  The lanes of one cell examine the
+// near lines in a rotated order (lane rank r0 within the cell's run of
+// lanes starts at line r0) and share what they found through LDS after each
+// step, so a near cell found adjacent by one lane is not tested again by the
+// others.  A near cell is skipped when it is heavy (the heavy-cell pass sets
+// the bit), is already known adjacent, or its box lies beyond eps of p; it
+// is adjacent without a test when its whole box lies inside the guard band
+// around p and it holds a core point; otherwise its core points are tested
+// until one is joined to p.  The box distances use the same fixed-point
+// arithmetic as the distance, so they bound it.
 template <int D, bool F32>
-__device__ unsigned adj_near_line(const Dev& d, const Inst& I, const Par& P, int32_t A, int r,
-                                  int32_t qs, int32_t qe, bool allc, Tally& t,
-                                  long long& freeb) {
-  LineCells L;
-  line_cells<D>(d, I, A, r, L);
-  const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
-  const int o1 = D == 3 ? kRowOff3[r][1] : 0;
-  unsigned todo = 0u;
+__device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, int32_t c,
+                         bool act, int head, int rank, unsigned long long* xm,
+                         unsigned long long& m0, Tally& t, long long& freeb) {
+  constexpr int NN = Near<D>::lines;
+  constexpr int B = Batch<F32>::value;
+  constexpr uint32_t kFrac = (1u << kFixBits) - 1u;
+  uint32_t pg[D];
+  double px[D];
+  int32_t fr[D];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    const int ol = L.ol[k];
-    if (k < L.len && L.st + k != A && ol >= -1 && ol <= 1 && L.nco[k] > 0 &&
-        L.b1[k] - L.b0[k] <= kHeavy)
-      todo |= 1u << k;
+  for (int k = 0; k < D; ++k) pg[k] = act ? d.G[k][p] : 0u;
+#pragma unroll
+  for (int k = 0; k < D; ++k) px[k] = (!F32 && act) ? d.X[k][p] : 0.0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) fr[k] = (int32_t)(pg[k] & kFrac);
+  const int64_t keyA = act ? (int64_t)d.cell_key[c] : 0;
+  const int64_t nkl = I.nk[D - 1];
+  const int64_t nk1 = I.nk[1];
+  const int32_t* rows = d.rows + (int64_t)c * kMaxRows;
+  for (int step = 0; step < NN; ++step) {
+    const int r = (step + rank) % NN;
+    const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
+    const int o1 = D == 3 ? kRowOff3[r][1] : 0;
+    const int32_t pk = act ? rows[r] : 0;
+    const int32_t len = pk >> kRowCntShift;
+    if (len > 0) {
+      const int32_t st = pk & kRowCellMask;
+      int32_t bnd[6], nco[5];
+      int64_t key[5];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) bnd[k] = k <= len ? d.cell_start[st + k] : 0;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        key[k] = k < len ? (int64_t)d.cell_key[st + k] : 0;
+        nco[k] = k < len ? d.cell_ncore[st + k] : 0;
+      }
+      const int64_t line_delta = D == 3 ? ((int64_t)o0 * nk1 + o1) * nkl : (int64_t)o0 * nkl;
+      float g2 = 0.f, h2 = 0.f;  // over the line's axes
+      {
+        const float a = cell_gap(o0, fr[0]), fa = cell_far(o0, fr[0]);
+        g2 = a * a;
+        h2 = fa * fa;
+        if (D == 3) {
+          const float b = cell_gap(o1, fr[1]), fb = cell_far(o1, fr[1]);
+          g2 = __builtin_fmaf(b, b, g2);
+          h2 = __builtin_fmaf(fb, fb, h2);
+        }
+      }
+      int32_t aa[5], ll[5];
+      const int base = 5 * r;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const int32_t n = k < len ? bnd[k + 1] - bnd[k] : 0;
+        const int ol = (int)(key[k] - keyA - line_delta);  // offset along the last axis
+        const float e = cell_gap(ol, fr[D - 1]), fe = cell_far(ol, fr[D - 1]);
+        const bool reach = !(__builtin_fmaf(e, e, g2) > P.d2_out);
+        const bool inside = __builtin_fmaf(fe, fe, h2) < P.d2_in;
+        bool use = k < len && ol >= -1 && ol <= 1 && nco[k] > 0 && n <= kHeavy && st + k != c &&
+                   reach && !((m0 >> (base + k)) & 1ull);
+        if (use && inside) {
+          // every point of the cell is joined to p, and one of them is core
+          m0 |= 1ull << (base + k);
+          ++freeb;
+          use = false;
+        }
+        aa[k] = bnd[k];
+        ll[k] = use ? n : 0;
+      }
+      Stream5 s;
+      s.set(aa, ll);
+      for (int32_t t0 = 0; t0 < s.tot; t0 += B) {
+        int32_t qi[B];
+        int kk[B];
+        bool cf[B];
+        uint32_t cg[B][D];
+        double cx[B][D];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+          int jr = 0;
+          const bool in = t0 + j < s.tot;
+          qi[j] = in ? s.at(t0 + j, jr) : p;
+          kk[j] = jr;
+          const bool need = in && !((m0 >> (base + jr)) & 1ull);
+          cf[j] = need && d.core[qi[j]];
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            if (F32)
+              cg[j][k] = need ? d.G[k][qi[j]] : 0u;
+            else
+              cx[j][k] = need ? d.X[k][qi[j]] : 0.0;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+          if (cf[j] && !((m0 >> (base + kk[j])) & 1ull)) {
+            ++t.eval;
+            if (edge_pair<D, F32>(d, P, p, qi[j], pg, cg[j], px, cx[j], t))
+              m0 |= 1ull << (base + kk[j]);
+          }
+        }
+      }
+    }
+    // share this step's finds with the other lanes of the cell
+    if (act && m0) atomicOr(&xm[head], m0);
+    wave_sync();
+    if (act) m0 |= xm[head];
   }
-  // one copy of the decision: the chosen cell's data is read again (cache
-  // hits) instead of being selected from register arrays
-  unsigned bits = 0u;
-  while (todo) {
-    const int k = __ffs(todo) - 1;
-    todo &= todo - 1;
-    const int32_t B = L.st + k;
-    const int ol = cell_offset_last<D>(d, I, A, B, r);
-    if (cells_joined<D, F32>(d, P, qs, qe, allc, d.cell_start[B], d.cell_start[B + 1], o0, o1,
-                             ol, t, freeb))
-      bits |= 1u << k;
-  }
-  return bits;
 }
 
 template <bool F32>
 __global__ __launch_bounds__(256) void k_adj(Dev d) {
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t A64 = tid >> 4;
-  const int r = (int)(tid & 15);
-  const int64_t C = d.ctl[C_NCELL];
-  const int32_t A = A64 < C ? (int32_t)A64 : 0;
-  int32_t qs = 0, qe = 0, ncA = 0;
+  __shared__ unsigned long long s_m[4][64];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  const bool act = i < d.ctl[C_NADJPTS];
+  const int32_t p = act ? d.adj_pts[i] : 0;
+  const int32_t c = act ? d.cell_of[p] : -1;
+  // the lanes of one cell form a contiguous run; head = its first lane here
+  const int32_t cprev = __shfl_up(c, 1, 64);
+  const unsigned long long mh = ballot(act && (lane == 0 || cprev != c));
+  const unsigned long long below = lane == 63 ? mh : (mh & ((2ull << lane) - 1ull));
+  const int head = below ? 63 - __clzll((long long)below) : 0;
+  const int rank = lane - head;
+  s_m[wv][lane] = 0ull;
+  wave_sync();
   int dim = 0;
   const Inst* I = d.inst;
-  if (A64 < C) {
-    qs = d.cell_start[A];
-    qe = d.cell_start[A + 1];
-    ncA = d.cell_ncore[A];
-    I = d.inst + d.cell_inst[A];
+  if (act) {
+    I = inst_of_cell(d, c);
     dim = I->dim;
   }
-  const bool light = A64 < C && ncA > 0 && qe - qs <= kHeavy;
-  const bool act = light && r < (dim == 3 ? Near<3>::lines : Near<2>::lines);
   Tally t;
   long long freeb = 0;
-  unsigned bits = 0u;
+  unsigned long long m0 = 0;
   const Par P(*I);
-  const bool allc = ncA == qe - qs;
-  DCLUST_BY_DIM(act, dim, (bits = adj_near_line<3, F32>(d, *I, P, A, r, qs, qe, allc, t, freeb)),
-                (bits = adj_near_line<2, F32>(d, *I, P, A, r, qs, qe, allc, t, freeb)));
-  // OR the lines' bits (slot 5r+k, r < 9) over the 16 lanes of the cell
-  unsigned long long m0 = (unsigned long long)bits << (5 * r);
-#pragma unroll
-  for (int o = 1; o < 16; o <<= 1) m0 |= __shfl_xor(m0, o, 64);
-  // light core cells own their mask here (zeroed by k_core; the heavy pass
-  // only ORs mirrored bits into it afterwards)
-  if (light && r == 0) d.amask[2 * (int64_t)A] = m0;
+  // every lane runs the line loop (inactive lanes only take part in the
+  // exchange), so the exchange points are uniform over the wave
+  const bool any3 = ballot(act && dim == 3) != 0, any2 = ballot(act && dim == 2) != 0;
+  if (any3) adj_lane<3, F32>(d, *I, P, p, c, act && dim == 3, head, rank, s_m[wv], m0, t, freeb);
+  if (any2) adj_lane<2, F32>(d, *I, P, p, c, act && dim == 2, head, rank, s_m[wv], m0, t, freeb);
+  // one atomic per cell run: the exchange slot of its head lane holds the
+  // union of the run's finds
+  if (act && lane == head) {
+    const unsigned long long u = s_m[wv][lane];
+    if (u) atomicOr(&d.amask[2 * (int64_t)c], u);
+  }
   wave_add(&d.ctl[C_BAND], t.band);
   wave_add(&d.ctl[C_ASYM], t.asym);
   wave_add(&d.ctl[C_ADJ_TESTS], t.eval);
@@ -738,7 +819,7 @@ __global__ __launch_bounds__(64) void k_adj_heavy(Dev d) {
   Tally t;
   for (long long h = blockIdx.x; h < nh; h += gridDim.x) {
     const int32_t A = d.heavy[h];
-    const Inst& I = d.inst[d.cell_inst[A]];
+    const Inst& I = *inst_of_cell(d, A);
     const Par P(I);
     if (I.dim == 3)
       adj_heavy<3, F32>(d, I, P, A, s_qi, s_qg, s_qx, t);
@@ -751,48 +832,65 @@ __global__ __launch_bounds__(64) void k_adj_heavy(Dev d) {
 }
 
 // ------------------------------------------------------------ far pairs --
-// Component of every core cell after the expansion and merge: the root of
-// its chain.
-__global__ __launch_bounds__(256) void k_cell_comp(Dev d) {
+// Component of every core cell after the expansion and merge (the root of
+// its chain), and the compacted list of core cells.
+__global__ __launch_bounds__(1024) void k_cell_comp(Dev d) {
   const int64_t C = d.ctl[C_NCELL];
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < C;
-       c += (int64_t)gridDim.x * blockDim.x)
-    d.cell_comp[c] = d.cell_ncore[c] > 0 ? d.parent[d.cell_label[c]] : -1;
+  for (int64_t c0 = blockIdx.x * (int64_t)blockDim.x; c0 < C;
+       c0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = c0 + threadIdx.x;
+    const bool in = c < C;
+    const bool hc = in && d.cell_ncore[c] > 0;
+    if (in) d.cell_comp[c] = hc ? d.parent[d.cell_label[c]] : -1;
+    block_append(hc, (int32_t)c, d.core_cells, &d.ctl[C_NCORECELLS]);
+  }
 }
 
-// Core cell A, stencil line r (one lane; 32 lanes per cell): its far cells
-// with core points in another component are decided, each unordered pair by
-// its lower cell; a joined pair unions the two components (lock-free
-// union-find on the chain forest).
+// Core cell A, one lane: every far stencil cell above A (each unordered pair
+// is taken by its lower cell) with core points in another component is
+// decided; a joined pair unions the two components (lock-free union-find on
+// the chain forest).
 template <int D, bool F32>
-__device__ long long far_line(const Dev& d, const Inst& I, const Par& P, int32_t A, int r,
-                              int32_t qs, int32_t qe, bool allc, int32_t compA, Tally& t,
+__device__ long long far_cell(const Dev& d, const Inst& I, const Par& P, int32_t A, Tally& t,
                               long long& freeb) {
-  LineCells L;
-  line_cells<D>(d, I, A, r, L);
-  const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
-  const int o1 = D == 3 ? kRowOff3[r][1] : 0;
-  const bool far_l = o0 == 2 || o0 == -2 || o1 == 2 || o1 == -2;
-  int32_t comp[5];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) comp[k] = k < L.len ? d.cell_comp[L.st + k] : compA;
-  unsigned todo = 0u;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    const int ol = L.ol[k];
-    const bool far = far_l || ol == 2 || ol == -2;
-    if (k < L.len && far && L.st + k > A && L.nco[k] > 0 && comp[k] != compA) todo |= 1u << k;
-  }
+  constexpr int NR = D == 3 ? 25 : 5;
+  const int32_t qs = d.cell_start[A], qe = d.cell_start[A + 1];
+  const bool allc = d.cell_ncore[A] == qe - qs;
+  const int32_t compA = d.cell_comp[A];
+  const int64_t keyA = (int64_t)d.cell_key[A];
+  const int64_t nkl = I.nk[D - 1];
+  const int32_t* rows = d.rows + (int64_t)A * kMaxRows;
   long long joins = 0;
-  while (todo) {
-    const int k = __ffs(todo) - 1;
-    todo &= todo - 1;
-    const int32_t B = L.st + k;
-    const int ol = cell_offset_last<D>(d, I, A, B, r);
-    if (cells_joined<D, F32>(d, P, qs, qe, allc, d.cell_start[B], d.cell_start[B + 1], o0, o1,
-                             ol, t, freeb)) {
-      uf_union(d.parent, compA, d.cell_comp[B]);
-      ++joins;
+  for (int r = 0; r < NR; ++r) {
+    const int32_t pk = rows[r];
+    const int32_t len = pk >> kRowCntShift;
+    if (len == 0) continue;
+    const int32_t st = pk & kRowCellMask;
+    if (st + len - 1 <= A) continue;  // no cell above A in this line
+    const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
+    const int o1 = D == 3 ? kRowOff3[r][1] : 0;
+    const bool far_l = o0 == 2 || o0 == -2 || o1 == 2 || o1 == -2;
+    const int64_t line_delta = D == 3 ? ((int64_t)o0 * I.nk[1] + o1) * nkl : (int64_t)o0 * nkl;
+    unsigned todo = 0u;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const int32_t B = st + k;
+      const bool up = k < len && B > A;
+      const int32_t nco = up ? d.cell_ncore[B] : 0;
+      const int32_t comp = up ? d.cell_comp[B] : compA;
+      const int ol = (up && !far_l) ? (int)((int64_t)d.cell_key[B] - keyA - line_delta) : 2;
+      if (up && (ol == 2 || ol == -2) && nco > 0 && comp != compA) todo |= 1u << k;
+    }
+    while (todo) {
+      const int k = __ffs(todo) - 1;
+      todo &= todo - 1;
+      const int32_t B = st + k;
+      const int ol = cell_offset_last<D>(d, I, A, B, r);
+      if (cells_joined<D, F32>(d, P, qs, qe, allc, d.cell_start[B], d.cell_start[B + 1], o0, o1,
+                               ol, t, freeb)) {
+        uf_union(d.parent, compA, d.cell_comp[B]);
+        ++joins;
+      }
     }
   }
   return joins;
@@ -800,30 +898,20 @@ __device__ long long far_line(const Dev& d, const Inst& I, const Par& P, int32_t
 
 template <bool F32>
 __global__ __launch_bounds__(256) void k_far(Dev d) {
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t A64 = tid >> 5;
-  const int r = (int)(tid & 31);
-  const int64_t C = d.ctl[C_NCELL];
-  const int32_t A = A64 < C ? (int32_t)A64 : 0;
-  int32_t qs = 0, qe = 0, ncA = 0, compA = -1;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i < d.ctl[C_NCORECELLS];
+  const int32_t A = act ? d.core_cells[i] : 0;
   int dim = 0;
   const Inst* I = d.inst;
-  if (A64 < C) {
-    qs = d.cell_start[A];
-    qe = d.cell_start[A + 1];
-    ncA = d.cell_ncore[A];
-    compA = d.cell_comp[A];
-    I = d.inst + d.cell_inst[A];
+  if (act) {
+    I = inst_of_cell(d, A);
     dim = I->dim;
   }
-  const bool act = A64 < C && ncA > 0 && r < (dim == 3 ? 25 : 5);
   Tally t;
   long long freeb = 0, joins = 0;
   const Par P(*I);
-  const bool allc = ncA == qe - qs;
-  DCLUST_BY_DIM(act, dim,
-                (joins = far_line<3, F32>(d, *I, P, A, r, qs, qe, allc, compA, t, freeb)),
-                (joins = far_line<2, F32>(d, *I, P, A, r, qs, qe, allc, compA, t, freeb)));
+  DCLUST_BY_DIM(act, dim, (joins = far_cell<3, F32>(d, *I, P, A, t, freeb)),
+                (joins = far_cell<2, F32>(d, *I, P, A, t, freeb)));
   wave_add(&d.ctl[C_BAND], t.band);
   wave_add(&d.ctl[C_ASYM], t.asym);
   wave_add(&d.ctl[C_FAR_TESTS], t.eval);
@@ -1083,7 +1171,7 @@ __global__ void k_seg_init(Dev d, int64_t W) {
 // of thousands of chains), one workgroup merges every recorded pair on an
 // LDS copy of the chain forest with LDS atomics and writes the flattened
 // roots back.  Otherwise the pairs are merged in global memory.
-constexpr int kLdsChains = 16384;  // 64 KiB of LDS
+constexpr int kLdsChains = 32768;  // 128 KiB of LDS (gfx950 allows 160 KiB per workgroup)
 
 __device__ __forceinline__ int32_t lds_find(int32_t* par, int32_t x) {
   for (;;) {
@@ -1282,7 +1370,7 @@ __global__ __launch_bounds__(256) void k_border(Dev d) {
     c = d.cell_of[p];
     qs = d.cell_start[c];
     qe = d.cell_start[c + 1];
-    I = d.inst + d.cell_inst[c];
+    I = inst_of_cell(d, c);
     dim = I->dim;
   }
   Tally t;
@@ -1352,7 +1440,7 @@ __global__ __launch_bounds__(256) void k_output(Dev d) {
     if (i < d.n) {
       const int32_t o = d.orig[i];
       const int32_t k = d.keyv[i];
-      const Inst& I = d.inst[d.cell_inst[d.cell_of[i]]];
+      const Inst& I = *inst_of_cell(d, d.ninst == 1 ? 0 : d.cell_of[i]);
       const int32_t lab = k == kNoKey ? 0 : d.rank[k] - d.rank[I.pt_begin] + 1;
       d.labels_out[o] = lab;
       d.core_out[o] = d.core[i];
@@ -1370,13 +1458,6 @@ static int point_grid(int64_t n) {
   return (int)(b < 8 ? 8 : b);
 }
 
-// 32 lanes per cell, for up to n cells (the cell count is known on the
-// device only; lanes past it exit at once)
-static int cell_line_grid(int64_t n, int lanes) {
-  int64_t b = (n * lanes + 255) / 256;
-  return (int)(b < 1 ? 1 : b);
-}
-
 void launch_core(const Dev& d, bool f32, hipStream_t s) {
   if (f32)
     hipLaunchKernelGGL(k_core<true>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
@@ -1386,10 +1467,10 @@ void launch_core(const Dev& d, bool f32, hipStream_t s) {
 
 void launch_adj(const Dev& d, bool f32, int heavy_grid, hipStream_t s) {
   if (f32) {
-    hipLaunchKernelGGL(k_adj<true>, dim3(cell_line_grid(d.n, 16)), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_adj<true>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_adj_heavy<true>, dim3(heavy_grid), dim3(64), 0, s, d);
   } else {
-    hipLaunchKernelGGL(k_adj<false>, dim3(cell_line_grid(d.n, 16)), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_adj<false>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_adj_heavy<false>, dim3(heavy_grid), dim3(64), 0, s, d);
   }
 }
@@ -1427,11 +1508,11 @@ void launch_merge(const Dev& d, int grid, hipStream_t s) {
 }
 
 void launch_far(const Dev& d, bool f32, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_cell_comp, dim3(grid), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_cell_comp, dim3(grid / 4), dim3(1024), 0, s, d);
   if (f32)
-    hipLaunchKernelGGL(k_far<true>, dim3(cell_line_grid(d.n, 32)), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_far<true>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
   else
-    hipLaunchKernelGGL(k_far<false>, dim3(cell_line_grid(d.n, 32)), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_far<false>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_flatten_all, dim3(grid), dim3(256), 0, s, d);
 }
 

@@ -605,7 +605,7 @@ __global__ __launch_bounds__(256) void k_gather(Dev d) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < d.n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int32_t src = vsrc[i];
-    const Inst& I = d.inst[d.cell_inst[d.cell_of[i]]];
+    const Inst& I = *inst_of_cell(d, d.ninst == 1 ? 0 : d.cell_of[i]);
     const int dim = I.dim;
     const double* p = d.xyz + I.xyz_off + ((int64_t)src - I.pt_begin) * dim;
     for (int k = 0; k < dim; ++k) {
@@ -640,7 +640,7 @@ __device__ __forceinline__ int32_t lower_bound_key_in(const uint64_t* a, int32_t
 
 // line of a cell: all coordinates but the last; global line-table index
 __device__ __forceinline__ int64_t cell_line(const Dev& d, int32_t c) {
-  const Inst& I = d.inst[d.cell_inst[c]];
+  const Inst& I = *inst_of_cell(d, c);
   int32_t kc[3];
   unpack_cell(I.dim, d.cell_xyz[c], kc);
   return I.line_base + (I.dim == 3 ? (int64_t)kc[0] * I.nk[1] + kc[1] : (int64_t)kc[0]);
@@ -680,7 +680,7 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
        t += (int64_t)gridDim.x * blockDim.x) {
     const int32_t c = (int32_t)(t / kMaxRows);
     const int r = (int)(t - (int64_t)c * kMaxRows);
-    const Inst& I = d.inst[d.cell_inst[c]];
+    const Inst& I = *inst_of_cell(d, c);
     const int dim = I.dim;
     const int nrow = dim == 3 ? 25 : 5;
     int32_t packed = 0;
@@ -736,9 +736,11 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
             for (int j = 0; j < 5; ++j) b += kk[j] <= khi;
           }
           packed = (b > a) ? (a | ((b - a) << kRowCntShift)) : 0;
-          nbrc += b - a;
-          cand += (long long)(d.cell_start[c + 1] - d.cell_start[c]) *
-                  (d.cell_start[b] - d.cell_start[a]);
+          if (d.nbr_stats) {
+            nbrc += b - a;
+            cand += (long long)(d.cell_start[c + 1] - d.cell_start[c]) *
+                    (d.cell_start[b] - d.cell_start[a]);
+          }
         }
       }
     }

@@ -124,13 +124,15 @@ enum Ctl : int {
   C_NLINES,     // total line-table entries
   C_DENSE,      // cells resolved as all-core without distance tests
   C_NHEAVY,     // heavy cells (listed in Dev::heavy)
+  C_NADJPTS,    // core points of light cells (listed in Dev::adj_pts)
+  C_NCORECELLS, // cells with core points (listed in Dev::core_cells)
   C_EDGES,      // adjacency bits of the expanded cells
   C_RECOVF,     // non-core points whose neighbour record overflowed
   C_ADJ_TESTS,  // pair tests of the adjacency lanes
   C_ADJ_FREE,   // adjacency bits set without a test (cell inside the eps ball)
   C_FAR_TESTS,  // pair tests of the far-pair pass
   C_FAR_JOINS,  // far pairs that joined two components
-  C_NCTL = 36
+  C_NCTL = 40
 };
 
 // Sort-pass bookkeeping: pass p is active when p*8 < key bits; its scatter
@@ -167,10 +169,13 @@ struct Dev {
   unsigned long long* amask;  // [N][2] near adjacency: bit 5r+k <-> k-th cell of line r
   int32_t* cell_comp;   // [N] component (root chain) of each core cell, -1 otherwise
   int32_t* heavy;       // heavy cells (more than kHeavy points)
+  int32_t* adj_pts;     // core points of light cells (near-adjacency lanes)
+  int32_t* core_cells;  // cells with core points (far-pair lanes)
   int32_t* rows;        // [N][kMaxRows] packed neighbour lines (first cell, count)
   int32_t* cell_of;     // [N] per sorted point
   int2* lines;          // [line_cap] first / end cell of every possible line
   int64_t line_cap;     // 0: no line table, rows found by binary search
+  int32_t nbr_stats;    // 1: k_rows also sums the neighbourhood sizes
 
   // sorted point data
   double* X[3];
@@ -294,6 +299,12 @@ __host__ __device__ inline double unord_d(unsigned long long u) {
 __device__ __forceinline__ int xcd_block(int b, int G) {
   int per = G >> 3;
   return (b & 7) * per + (b >> 3);
+}
+
+// Instance of a cell (a run of one instance reads the table entry directly,
+// with no per-cell load).
+__device__ __forceinline__ const Inst* inst_of_cell(const Dev& d, int64_t c) {
+  return d.ninst == 1 ? d.inst : d.inst + d.cell_inst[c];
 }
 
 // Cell index along one axis, shared by key computation, gather and every

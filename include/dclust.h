@@ -95,8 +95,10 @@ typedef struct dclust_stats {
   int64_t n_collisions; /* collision pairs recorded                         */
   int64_t n_overflow;   /* points pushed past a full seed list (corrected)  */
   int64_t n_passes;     /* expansion passes                                 */
-  int64_t cand_total;   /* sum over cells of n_c * (points in 3^D nbhd)     */
-  int64_t nbr_cells_total; /* sum over cells of non-empty neighbour cells   */
+  int64_t cand_total;   /* sum over cells of n_c * (points in the 5^D
+                         * stencil); computed only when the environment
+                         * variable DCLUST_NBR_STATS=1 is set (extra loads) */
+  int64_t nbr_cells_total; /* sum over cells of non-empty stencil cells (same) */
   int64_t band_rechecks;   /* FP32 form: pairs re-decided in FP64           */
   int64_t band_flips;      /* FP32 form: rechecks where FP32 alone was wrong */
   int64_t asym_pairs;      /* core-core pairs with P(p,q) != P(q,p)         */
