@@ -47,7 +47,9 @@ struct dclust_handle {
   hipEvent_t ev[E_N] = {};
   std::vector<Buf> bufs;
   int64_t last_n = -1;
-  uint8_t* last_core = nullptr;  // device, input order
+  uint8_t* last_core = nullptr;        // device, cell-sorted core flags of the last run
+  const int32_t* last_orig = nullptr;  // device, cell-sorted input indices
+  uint8_t* last_core_out = nullptr;    // device, input-order staging
   int last_err = 0;
 
   template <class T>
@@ -367,7 +369,9 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     HIPCHK(hipStreamSynchronize(s));
     t_pass0_ms = ev_ms(h, E_ADJ, E_PASS0);
     h->last_n = ntot;
-    h->last_core = d.core_out;
+    h->last_core = d.core;
+    h->last_orig = d.orig;
+    h->last_core_out = d.core_out;
     if (stats) {
       stats->t_index_ms = ev_ms(h, E_START, E_CORE0);
       stats->t_core_ms = ev_ms(h, E_CORE0, E_CORE);
@@ -570,7 +574,13 @@ int dclust_run_batched(dclust_handle* h, int32_t count, const double* const* xyz
 
 int dclust_last_core_flags(dclust_handle* h, uint8_t* core_out, int64_t n) {
   if (!h || !core_out || n != h->last_n || !h->last_core) return DCLUST_E_PARAM;
-  if (hipMemcpy(core_out, h->last_core, (size_t)n, hipMemcpyDeviceToHost) != hipSuccess)
+  if (hipSetDevice(h->device) != hipSuccess) return DCLUST_E_HIP;
+  launch_core_out(h->last_core, h->last_orig, n, h->last_core_out, h->stream);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return DCLUST_E_HIP;
+  }
+  if (hipMemcpy(core_out, h->last_core_out, (size_t)n, hipMemcpyDeviceToHost) != hipSuccess)
     return DCLUST_E_HIP;
   return DCLUST_OK;
 }

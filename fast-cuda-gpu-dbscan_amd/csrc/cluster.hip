@@ -35,7 +35,8 @@
 //   k_merge_lds / k_union_pairs / k_flatten   GPU atomic union-find over
 //                chain ids (MonitorSeedPoints' CPU BFS over the collision
 //                matrix and thrust::replace relabels, dbscan.cu:60-125).
-//   k_compmin / k_keys / k_border / rank scan / k_output   canonical labels:
+//   k_cell_comp / k_flatten_all / k_keys / k_border / rank scan / k_output
+//                canonical labels:
 //                cluster id = rank of the component's minimum core input
 //                index; border = minimum id over adjacent cores; noise = 0
 //                (GetDbscanResult, dbscan.cu:225-251, writes sorted labels
@@ -841,7 +842,18 @@ __global__ __launch_bounds__(1024) void k_cell_comp(Dev d) {
     const int64_t c = c0 + threadIdx.x;
     const bool in = c < C;
     const bool hc = in && d.cell_ncore[c] > 0;
-    if (in) d.cell_comp[c] = hc ? d.parent[d.cell_label[c]] : -1;
+    if (hc) {
+      const int32_t chain = d.cell_label[c];
+      d.cell_comp[c] = d.parent[chain];
+      // the sort is stable, so input indices increase within a cell: the
+      // cell's smallest core index is that of its first core point
+      int32_t q = d.cell_start[c];
+      if (d.cell_ncore[c] != d.cell_start[c + 1] - q)
+        while (!d.core[q]) ++q;
+      atomicMin(&d.compmin[chain], d.orig[q]);
+    } else if (in) {
+      d.cell_comp[c] = -1;
+    }
     block_append(hc, (int32_t)c, d.core_cells, &d.ctl[C_NCORECELLS]);
   }
 }
@@ -918,12 +930,17 @@ __global__ __launch_bounds__(256) void k_far(Dev d) {
   wave_add(&d.ctl[C_FAR_JOINS], joins);
 }
 
-// Roots of every chain after the far-pair unions.
+// Roots of every chain after the far-pair unions, and the minimum core
+// input index of each component (the minimum over its chains).
 __global__ __launch_bounds__(256) void k_flatten_all(Dev d) {
   const long long nc = d.ctl[C_NCHAIN];
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc;
-       c += (long long)gridDim.x * blockDim.x)
-    d.parent[c] = uf_find(d.parent, (int32_t)c);
+       c += (long long)gridDim.x * blockDim.x) {
+    const int32_t r = uf_find(d.parent, (int32_t)c);
+    d.parent[c] = r;
+    const int32_t m = d.compmin[c];
+    if (m != kNoKey && r != c) atomicMin(&d.compmin[r], m);
+  }
 }
 
 // ---------------------------------------------------------------- expand --
@@ -1172,6 +1189,7 @@ __global__ void k_seg_init(Dev d, int64_t W) {
 // LDS copy of the chain forest with LDS atomics and writes the flattened
 // roots back.  Otherwise the pairs are merged in global memory.
 constexpr int kLdsChains = 32768;  // 128 KiB of LDS (gfx950 allows 160 KiB per workgroup)
+constexpr int kMergeChunk = 4096;  // pair buffers scanned at a time (16 KiB of LDS)
 
 __device__ __forceinline__ int32_t lds_find(int32_t* par, int32_t x) {
   for (;;) {
@@ -1188,18 +1206,50 @@ __device__ __forceinline__ int64_t pair_waves(const Dev& d) { return d.pair_cap 
 
 __global__ __launch_bounds__(1024) void k_merge_lds(Dev d) {
   __shared__ int32_t par[kLdsChains];
+  __shared__ int32_t s_pre[kMergeChunk + 1];
+  __shared__ int32_t s_wsum[16];
   const long long nc = d.ctl[C_NCHAIN];
   if (nc > kLdsChains) return;
   // parent may already hold inline merges (pairs past a full buffer)
   for (int i = threadIdx.x; i < nc; i += blockDim.x) par[i] = d.parent[i];
-  __syncthreads();
-  // each wavefront of the block takes one expansion wave's buffer at a time
-  // and merges its pairs across its lanes
+  // the pair buffers of kMergeChunk expansion waves at a time: their counts
+  // are scanned in LDS, then every thread takes pairs by flat index, so the
+  // buffers are read in parallel
   const int64_t nw = pair_waves(d);
-  for (int64_t w = threadIdx.x >> 6; w < nw; w += blockDim.x >> 6) {
-    const int cnt = min(d.pair_cnt[w], d.pair_per);
-    for (int k = lane_id(); k < cnt; k += 64) {
-      int2 pr = d.pairs[w * d.pair_per + k];
+  const int per = d.pair_per;
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  for (int64_t w0 = 0; w0 < nw; w0 += kMergeChunk) {
+    const int nb = (int)((nw - w0) < kMergeChunk ? (nw - w0) : kMergeChunk);
+    // per thread: kMergeChunk / 1024 consecutive buffers
+    constexpr int kPer = kMergeChunk / 1024;
+    int cnt[kPer];
+    int sum = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int b = threadIdx.x * kPer + j;
+      cnt[j] = b < nb ? min(d.pair_cnt[w0 + b], per) : 0;
+      sum += cnt[j];
+    }
+    const int incl = wave_incl_scan(sum);
+    if (lane == 63) s_wsum[wv] = incl;
+    __syncthreads();
+    int off = incl - sum;
+    for (int w = 0; w < wv; ++w) off += s_wsum[w];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      s_pre[threadIdx.x * kPer + j] = off;
+      off += cnt[j];
+    }
+    if (threadIdx.x == blockDim.x - 1) s_pre[kMergeChunk] = off;
+    __syncthreads();
+    const int tot = s_pre[kMergeChunk];
+    for (int f = threadIdx.x; f < tot; f += blockDim.x) {
+      int lo = 0, hi = kMergeChunk - 1;  // last buffer whose range starts at or before f
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pre[mid] <= f) lo = mid; else hi = mid - 1;
+      }
+      const int2 pr = d.pairs[(w0 + lo) * per + (f - s_pre[lo])];
       int32_t a = lds_find(par, pr.x), b = lds_find(par, pr.y);
       while (a != b) {
         if (a < b) {
@@ -1213,8 +1263,8 @@ __global__ __launch_bounds__(1024) void k_merge_lds(Dev d) {
         b = lds_find(par, b);
       }
     }
+    __syncthreads();  // s_pre is rewritten by the next chunk
   }
-  __syncthreads();
   for (int i = threadIdx.x; i < nc; i += blockDim.x) {
     int32_t r = i;
     while (par[r] != r) r = par[r];
@@ -1247,28 +1297,6 @@ __global__ __launch_bounds__(256) void k_flatten(Dev d) {
 // chain of a core point: the chain that claimed its cell
 __device__ __forceinline__ int32_t core_chain(const Dev& d, int64_t i) {
   return d.core[i] ? d.cell_label[d.cell_of[i]] : -1;
-}
-
-// minimum input index per component (wave-aggregated per distinct root)
-__global__ __launch_bounds__(256) void k_compmin(Dev d) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); i0 < d.n; i0 += stride) {
-    const int64_t i = i0 + lane_id();
-    const int32_t li = i < d.n ? core_chain(d, i) : -1;
-    bool cr = li >= 0;  // every core cell is claimed; the guard only keeps a bug in bounds
-    const int32_t r = cr ? d.parent[li] : -1;
-    const int32_t o = cr ? d.orig[i] : kNoKey;
-    unsigned long long m = ballot(cr);
-    while (m) {
-      const int l = __ffsll((long long)m) - 1;
-      const int32_t rr = rl_i(r, l);
-      const bool mine = cr && r == rr;
-      const int32_t mn = wave_min(mine ? o : kNoKey);
-      if (lane_id() == 0) atomicMin(&d.compmin[rr], mn);
-      cr = cr && !mine;
-      m = ballot(cr);
-    }
-  }
 }
 
 __global__ __launch_bounds__(256) void k_keys(Dev d) {
@@ -1443,7 +1471,6 @@ __global__ __launch_bounds__(256) void k_output(Dev d) {
       const Inst& I = *inst_of_cell(d, d.ninst == 1 ? 0 : d.cell_of[i]);
       const int32_t lab = k == kNoKey ? 0 : d.rank[k] - d.rank[I.pt_begin] + 1;
       d.labels_out[o] = lab;
-      d.core_out[o] = d.core[i];
       noise += lab == 0;
     }
   }
@@ -1520,12 +1547,27 @@ void launch_border(const Dev& d, bool f32, hipStream_t s) {
   int g2 = (int)((d.n + 255) / 256);
   if (g2 > 4096) g2 = 4096;
   if (g2 < 1) g2 = 1;
-  hipLaunchKernelGGL(k_compmin, dim3(g2), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_keys, dim3(g2), dim3(256), 0, s, d);
   if (f32)
     hipLaunchKernelGGL(k_border<true>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
   else
     hipLaunchKernelGGL(k_border<false>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
+}
+
+// core flags to input order (on request, after a run)
+__global__ __launch_bounds__(256) void k_core_out(const uint8_t* core, const int32_t* orig,
+                                                  int64_t n, uint8_t* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[orig[i]] = core[i];
+}
+
+void launch_core_out(const uint8_t* core, const int32_t* orig, int64_t n, uint8_t* out,
+                     hipStream_t s) {
+  int g = (int)((n + 255) / 256);
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL(k_core_out, dim3(g), dim3(256), 0, s, core, orig, n, out);
 }
 
 void launch_relabel(const Dev& d, int64_t n, hipStream_t s) {

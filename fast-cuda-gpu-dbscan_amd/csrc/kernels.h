@@ -34,6 +34,9 @@ void launch_merge(const Dev& d, int grid, hipStream_t s);
 void launch_far(const Dev& d, bool f32, int grid, hipStream_t s);
 void launch_border(const Dev& d, bool f32, hipStream_t s);
 void launch_relabel(const Dev& d, int64_t n, hipStream_t s);
+// core flags of the last run (cell-sorted core[], orig[]) to input order
+void launch_core_out(const uint8_t* core, const int32_t* orig, int64_t n, uint8_t* out,
+                     hipStream_t s);
 
 // occupancy helper for the expansion kernel
 int expand_blocks_per_cu(int seeds, int threads);
