@@ -225,6 +225,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
   d.xyz = d_xyz;
   d.pair_cap = shape.pair_cap;
   d.pair_per = shape.pair_per;
+  d.rstride = ntot;
   // neighbourhood sizes (stats.cand_total / nbr_cells_total) cost an extra
   // dependent load per stencil line; they are summed only on request
   {

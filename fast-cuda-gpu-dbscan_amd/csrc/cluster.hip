@@ -209,11 +209,11 @@ struct Stream5 {
 };
 
 // point ranges of five stencil lines
-__device__ __forceinline__ void line_group(const Dev& d, const int32_t* rows5, int32_t* a,
+__device__ __forceinline__ void line_group(const Dev& d, const RowsOf& rows, int g, int32_t* a,
                                            int32_t* l) {
   int32_t pk[5];
 #pragma unroll
-  for (int j = 0; j < 5; ++j) pk[j] = rows5[j];
+  for (int j = 0; j < 5; ++j) pk[j] = rows[g + j];
 #pragma unroll
   for (int j = 0; j < 5; ++j) {
     const int32_t len = pk[j] >> kRowCntShift, st = pk[j] & kRowCellMask;
@@ -303,10 +303,10 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
   int32_t* rec = d.rec + (int64_t)p * R;
   int nrec = 0;
   long long ev = 0;
-  const int32_t* rows = d.rows + (int64_t)c * kMaxRows;
+  const RowsOf rows = rows_of(d, c);
   for (int g = 0; g < NR && cnt < P.minpts; g += 5) {
     int32_t aa[5], ll[5];
-    line_group(d, rows + g, aa, ll);
+    line_group(d, rows, g, aa, ll);
     Stream5 s;
     s.set(aa, ll);
     for (int32_t t0 = 0; t0 < s.tot && cnt < P.minpts; t0 += B) {
@@ -550,7 +550,7 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
   const int64_t keyA = act ? (int64_t)d.cell_key[c] : 0;
   const int64_t nkl = I.nk[D - 1];
   const int64_t nk1 = I.nk[1];
-  const int32_t* rows = d.rows + (int64_t)c * kMaxRows;
+  const RowsOf rows = rows_of(d, c);
   for (int step = 0; step < NN; ++step) {
     const int r = (step + rank) % NN;
     const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
@@ -737,7 +737,7 @@ __device__ void adj_heavy(const Dev& d, const Inst& I, const Par& P, int32_t A, 
   wave_sync();
   const bool cached = nq <= kQCache;
   const int NQ = cached ? nq : (qe - qs);
-  const int32_t pkl = lane < NN ? d.rows[(int64_t)A * kMaxRows + lane] : 0;
+  const int32_t pkl = lane < NN ? rows_of(d, A)[lane] : 0;
   const int64_t keyA = (int64_t)d.cell_key[A];
   const int64_t nkl = I.nk[D - 1];
   unsigned long long m0 = 0;
@@ -801,7 +801,7 @@ __device__ void adj_heavy(const Dev& d, const Inst& I, const Par& P, int32_t A, 
         m0 |= 1ull << (5 * r + k);
         if (b1 - b0 <= kHeavy) {
           const int rm = D == 3 ? kRowMirror3[r] : kRowMirror2[r];
-          const int32_t pm = d.rows[(int64_t)Bc * kMaxRows + rm];
+          const int32_t pm = rows_of(d, Bc)[rm];
           const int bm = 5 * rm + (A - (pm & kRowCellMask));
           if (lane == 0) atomicOr(&d.amask[2 * (int64_t)Bc], 1ull << bm);
         }
@@ -842,17 +842,29 @@ __global__ __launch_bounds__(1024) void k_cell_comp(Dev d) {
     const int64_t c = c0 + threadIdx.x;
     const bool in = c < C;
     const bool hc = in && d.cell_ncore[c] > 0;
+    int32_t chain = -1, cmin = kNoKey;
     if (hc) {
-      const int32_t chain = d.cell_label[c];
+      chain = d.cell_label[c];
       d.cell_comp[c] = d.parent[chain];
       // the sort is stable, so input indices increase within a cell: the
       // cell's smallest core index is that of its first core point
       int32_t q = d.cell_start[c];
       if (d.cell_ncore[c] != d.cell_start[c + 1] - q)
         while (!d.core[q]) ++q;
-      atomicMin(&d.compmin[chain], d.orig[q]);
+      cmin = d.orig[q];
     } else if (in) {
       d.cell_comp[c] = -1;
+    }
+    // chain minimum: one atomic per distinct chain of the wave
+    bool pend = hc;
+    unsigned long long m = ballot(pend);
+    while (m) {
+      const int32_t lead = rl_i(chain, __ffsll((long long)m) - 1);
+      const bool mine = pend && chain == lead;
+      const int32_t v = wave_min(mine ? cmin : kNoKey);
+      if (lane_id() == 0) atomicMin(&d.compmin[lead], v);
+      pend = pend && !mine;
+      m = ballot(pend);
     }
     block_append(hc, (int32_t)c, d.core_cells, &d.ctl[C_NCORECELLS]);
   }
@@ -871,7 +883,7 @@ __device__ long long far_cell(const Dev& d, const Inst& I, const Par& P, int32_t
   const int32_t compA = d.cell_comp[A];
   const int64_t keyA = (int64_t)d.cell_key[A];
   const int64_t nkl = I.nk[D - 1];
-  const int32_t* rows = d.rows + (int64_t)A * kMaxRows;
+  const RowsOf rows = rows_of(d, A);
   long long joins = 0;
   for (int r = 0; r < NR; ++r) {
     const int32_t pk = rows[r];
@@ -1056,7 +1068,7 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
       m1 = d.amask[2 * (int64_t)A + 1];
     }
     edges += __popcll(m0) + __popcll(m1);
-    const int32_t* rowsA = d.rows + (int64_t)A * kMaxRows;
+    const RowsOf rowsA = rows_of(d, A);
     while (ballot((m0 | m1) != 0ull)) {
       constexpr int T = 8;  // neighbours per lane step
       int32_t tg[T];
@@ -1368,10 +1380,10 @@ __device__ int32_t border_lane(const Dev& d, const Par& P, int32_t p, int32_t c,
     return best;
   }
   if (R > 0) ++ovf;
-  const int32_t* rows = d.rows + (int64_t)c * kMaxRows;
+  const RowsOf rows = rows_of(d, c);
   for (int g = 0; g < NR; g += 5) {
     int32_t aa[5], ll[5];
-    line_group(d, rows + g, aa, ll);
+    line_group(d, rows, g, aa, ll);
     Stream5 s;
     s.set(aa, ll);
     for (int32_t i = 0; i < s.tot; ++i) {

@@ -676,75 +676,84 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
   const int32_t C = (int32_t)d.ctl[C_NCELL];
   const bool table = d.ctl[C_NLINES] <= d.line_cap;
   long long cand = 0, nbrc = 0;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)C * kMaxRows;
+  // one lane per (group of 5 lines, cell), cells fastest: the lanes of a
+  // wave are consecutive cells, which mostly share their neighbour lines
+  // (broadcast loads), and each lane has five lines' loads in flight
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)C * 5;
        t += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t c = (int32_t)(t / kMaxRows);
-    const int r = (int)(t - (int64_t)c * kMaxRows);
+    const int g = (int)(t / C);
+    const int32_t c = (int32_t)(t - (int64_t)g * C);
     const Inst& I = *inst_of_cell(d, c);
     const int dim = I.dim;
-    const int nrow = dim == 3 ? 25 : 5;
-    int32_t packed = 0;
-    if (r < nrow) {
-      int32_t kc[3];
-      unpack_cell(dim, d.cell_xyz[c], kc);
-      const int last = dim - 1;
-      int o0, o1 = 0;
+    const int last = dim - 1;
+    int32_t kc[3];
+    unpack_cell(dim, d.cell_xyz[c], kc);
+    const int32_t lo_l = kc[last] >= 2 ? kc[last] - 2 : 0;
+    const int32_t hi_l = kc[last] + 2 < I.nk[last] ? kc[last] + 2 : I.nk[last] - 1;
+    bool ok[5];
+    uint64_t klo[5], khi[5];
+    int32_t ls[5], le[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const int r = 5 * g + j;
+      int o0 = 0, o1 = 0;
       if (dim == 3) {
         o0 = kRowOff3[r][0];
         o1 = kRowOff3[r][1];
       } else {
-        o0 = kRowOff2[r];
+        o0 = r < 5 ? kRowOff2[r] : 0;
       }
       const int32_t r0 = kc[0] + o0, r1 = kc[1] + o1;
-      bool ok = r0 >= 0 && r0 < I.nk[0];
-      if (dim == 3) ok = ok && r1 >= 0 && r1 < I.nk[1];
-      if (ok) {
-        const int32_t lo_l = kc[last] >= 2 ? kc[last] - 2 : 0;
-        const int32_t hi_l = kc[last] + 2 < I.nk[last] ? kc[last] + 2 : I.nk[last] - 1;
-        const int64_t line = dim == 3 ? (int64_t)r0 * I.nk[1] + r1 : (int64_t)r0;
-        const int64_t base = line * I.nk[last];
-        const uint64_t klo = (uint64_t)(I.key_base + base + lo_l);
-        const uint64_t khi = (uint64_t)(I.key_base + base + hi_l);
-        int32_t ls = 0, le = C;
-        if (table) {
-          const int2 lr = d.lines[I.line_base + line];
-          ls = lr.x;
-          le = lr.y;
-        }
-        if (ls >= 0) {
-          // cells of the line with keys in [klo, khi] (at most 5): a short
-          // line is read in one go, a long one searched
-          int32_t a, b = 0;
-          if (le - ls <= 8) {
-            uint64_t kk[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) kk[j] = ls + j < le ? d.cell_key[ls + j] : ~0ull;
-            a = ls;
-            b = ls;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-              a += kk[j] < klo;
-              b += kk[j] <= khi;
-            }
-          } else {
-            a = lower_bound_key_in(d.cell_key, ls, le, klo);
-            uint64_t kk[5];
-#pragma unroll
-            for (int j = 0; j < 5; ++j) kk[j] = a + j < le ? d.cell_key[a + j] : ~0ull;
-            b = a;
-#pragma unroll
-            for (int j = 0; j < 5; ++j) b += kk[j] <= khi;
-          }
-          packed = (b > a) ? (a | ((b - a) << kRowCntShift)) : 0;
-          if (d.nbr_stats) {
-            nbrc += b - a;
-            cand += (long long)(d.cell_start[c + 1] - d.cell_start[c]) *
-                    (d.cell_start[b] - d.cell_start[a]);
-          }
-        }
+      ok[j] = (dim == 3 || g == 0) && r0 >= 0 && r0 < I.nk[0];
+      if (dim == 3) ok[j] = ok[j] && r1 >= 0 && r1 < I.nk[1];
+      const int64_t line = dim == 3 ? (int64_t)r0 * I.nk[1] + r1 : (int64_t)r0;
+      const int64_t base = line * I.nk[last];
+      klo[j] = (uint64_t)(I.key_base + base + lo_l);
+      khi[j] = (uint64_t)(I.key_base + base + hi_l);
+      ls[j] = 0;
+      le[j] = C;
+      if (ok[j] && table) {
+        const int2 lr = d.lines[I.line_base + line];
+        ls[j] = lr.x;
+        le[j] = lr.y;
       }
     }
-    d.rows[t] = packed;
+    // cells of each line with keys in [klo, khi] (at most 5): a line of at
+    // most 4 cells is read in one go, a longer one searched
+    uint64_t kk[5][4];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        kk[j][i] = (ok[j] && ls[j] >= 0 && le[j] - ls[j] <= 4 && ls[j] + i < le[j])
+                       ? d.cell_key[ls[j] + i]
+                       : ~0ull;
+    int32_t packed[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      packed[j] = 0;
+      if (!ok[j] || ls[j] < 0) continue;
+      int32_t a = ls[j], b = ls[j];
+      if (le[j] - ls[j] <= 4) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          a += kk[j][i] < klo[j];
+          b += kk[j][i] <= khi[j];
+        }
+      } else {
+        a = lower_bound_key_in(d.cell_key, ls[j], le[j], klo[j]);
+        b = a;
+        for (int i = 0; i < 5 && a + i < le[j]; ++i) b += d.cell_key[a + i] <= khi[j];
+      }
+      packed[j] = (b > a) ? (a | ((b - a) << kRowCntShift)) : 0;
+      if (d.nbr_stats) {
+        nbrc += b - a;
+        cand += (long long)(d.cell_start[c + 1] - d.cell_start[c]) *
+                (d.cell_start[b] - d.cell_start[a]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) d.rows[(int64_t)(5 * g + j) * d.rstride + c] = packed[j];
   }
   wave_add(&d.ctl[C_CAND], cand);
   wave_add(&d.ctl[C_NBRC], nbrc);

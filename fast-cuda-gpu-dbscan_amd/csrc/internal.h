@@ -171,7 +171,9 @@ struct Dev {
   int32_t* heavy;       // heavy cells (more than kHeavy points)
   int32_t* adj_pts;     // core points of light cells (near-adjacency lanes)
   int32_t* core_cells;  // cells with core points (far-pair lanes)
-  int32_t* rows;        // [N][kMaxRows] packed neighbour lines (first cell, count)
+  int32_t* rows;        // [kMaxRows][rstride] packed neighbour lines (first cell, count),
+                        // line-major so that consecutive cells are adjacent
+  int64_t rstride;      // >= number of cells
   int32_t* cell_of;     // [N] per sorted point
   int2* lines;          // [line_cap] first / end cell of every possible line
   int64_t line_cap;     // 0: no line table, rows found by binary search
@@ -299,6 +301,16 @@ __host__ __device__ inline double unord_d(unsigned long long u) {
 __device__ __forceinline__ int xcd_block(int b, int G) {
   int per = G >> 3;
   return (b & 7) * per + (b >> 3);
+}
+
+// The stencil lines of one cell: rows_of(d, c)[r] is line r (packed).
+struct RowsOf {
+  const int32_t* base;
+  int64_t stride;
+  __device__ __forceinline__ int32_t operator[](int r) const { return base[(int64_t)r * stride]; }
+};
+__device__ __forceinline__ RowsOf rows_of(const Dev& d, int64_t c) {
+  return RowsOf{d.rows + c, d.rstride};
 }
 
 // Instance of a cell (a run of one instance reads the table entry directly,
