@@ -96,6 +96,18 @@ def b_nbr(st, dim, bytes_per_coord):
             16 * dim * st.get("band_rechecks_core", 0))
 
 
+def measured_traffic(workload, precision):
+    """HBM bytes per launch of k_core on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated by a plain copy),
+    or None when no such measurement is committed."""
+    path = os.path.join(ROOT, "profiles", f"k_core_{workload}_{precision}_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def spatial_sample(x, m):
     """Points inside the sub-box [0, a]^D holding about m points (same local
     density as the full input)."""
@@ -136,6 +148,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (a plain copy of known size, "
+                         "used to calibrate the HBM byte counters of a profiler run)")
     args = ap.parse_args()
     ws = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or ws != 1:
@@ -148,6 +163,13 @@ def main():
     n, dim = x.shape
     hip = Hip()
     hip.check(hip.L.hipSetDevice(0), "hipSetDevice")
+    if args.calib_copy > 0:
+        nb = args.calib_copy << 20
+        src, dst = hip.malloc(nb), hip.malloc(nb)
+        hip.check(hip.L.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), nb, 3), "hipMemcpy D2D")
+        hip.sync()
+        hip.free(src)
+        hip.free(dst)
     d_x = hip.malloc(x.nbytes)
     d_lab = hip.malloc(n * 4)
     hip.h2d(d_x, x)
@@ -194,7 +216,8 @@ def main():
                    "precision": "fp32+fp64 guard band" if bpc == 4 else "fp64"},
         "roofline": {"kernel": "k_core (neighbour count)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(args.workload, args.precision),
                      "algorithmic_bytes": int(bytes_core), "avg_launch_ms": round(t_core, 5)},
         "phases_ms": phases,
         "counters": {k: st[k] for k in ("n_cells", "n_core", "n_noise", "n_clusters", "n_chains",
