@@ -149,7 +149,9 @@ RunShape pick_shape(dclust_handle* h, const dclust_params& p, int64_t n, bool f3
   r.waves = r.blocks * waves_per_block;
   r.pair_per = p.extra_collision > 0 ? std::max(1, p.extra_collision / waves_per_block) : 512;
   r.pair_cap = (int64_t)r.pair_per * r.waves;
-  // cells a wave expands per pass before handing its list on
+  // cells a wave expands per pass before handing its list on (a long chain
+  // is shared out by the next pass; concurrent seeding already splits large
+  // clusters between waves, so the budget rarely binds)
   r.budget = 4096;
   if (const char* e = std::getenv("DCLUST_EXPAND_BUDGET")) r.budget = std::max(1, std::atoi(e));
   (void)n;
@@ -232,12 +234,12 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     const char* e = std::getenv("DCLUST_NBR_STATS");
     d.nbr_stats = (e && e[0] == '1') ? 1 : 0;
   }
-  // neighbour records of non-core points: minpts slots per point (a
-  // non-core point has at most minpts - 1 neighbours outside its own cell,
-  // plus pairs whose box test is asymmetric)
+  // neighbour records of non-core points: a non-core point has fewer than
+  // minpts - 1 neighbours outside its own cell (plus pairs whose box test is
+  // asymmetric); up to kRecSlots are kept, a longer record is rescanned
   int rec_stride = 0;
   for (int32_t i = 0; i < ninst; ++i) rec_stride = std::max(rec_stride, params[i].minpts);
-  if (rec_stride > kMaxRec) rec_stride = 0;
+  rec_stride = std::min(rec_stride, kRecSlots);
   d.rec_stride = rec_stride;
   {
     hipError_t& E = herr;
@@ -272,7 +274,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     d.orig = h->get<int32_t>(S_ORIG, ntot, &E);
     d.core = h->get<uint8_t>(S_CORE, ntot, &E);
     d.cell_label = h->get<int32_t>(S_LABEL, ntot, &E);
-    d.amask = h->get<unsigned long long>(S_AMASK, (size_t)ntot * 2, &E);
+    d.amask = h->get<unsigned long long>(S_AMASK, (size_t)ntot, &E);
     d.heavy = h->get<int32_t>(S_HEAVY, ntot / (kHeavy + 1) + 1, &E);
     d.rec_n = h->get<uint8_t>(S_RECN, ntot, &E);
     if (rec_stride > 0) d.rec = h->get<int32_t>(S_REC, (size_t)ntot * rec_stride, &E);
@@ -411,6 +413,8 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
       stats->diag[9] = ctl_h[C_ADJ_FREE];
       stats->diag[10] = ctl_h[C_FAR_TESTS];
       stats->diag[11] = ctl_h[C_FAR_JOINS];
+      stats->diag[12] = ctl_h[C_POPS_MAX];
+      stats->diag[13] = ctl_h[C_SEEDS_MAX];
     }
     if (npass >= kMaxExpandPasses) return DCLUST_E_CAPACITY;
     return rc;

@@ -300,7 +300,10 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
       hi[k] = px[k] + P.eps;
     }
   }
-  int32_t* rec = d.rec + (int64_t)p * R;
+  // records stay in registers until the point is known to be non-core
+  int32_t rk[kRecSlots];
+#pragma unroll
+  for (int i = 0; i < kRecSlots; ++i) rk[i] = 0;
   int nrec = 0;
   long long ev = 0;
   const RowsOf rows = rows_of(d, c);
@@ -353,7 +356,8 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
           }
           cnt += fwd;
           if (sym) {
-            if (nrec < R) rec[nrec] = q;
+#pragma unroll
+            for (int i = 0; i < kRecSlots; ++i) rk[i] = nrec == i ? q : rk[i];
             ++nrec;
           }
           ++ev;
@@ -362,7 +366,13 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
     }
   }
   isc = cnt >= P.minpts;
-  if (!isc && R > 0) d.rec_n[p] = (uint8_t)(nrec < 255 ? nrec : 255);
+  if (!isc) {
+    int32_t* rec = d.rec + (int64_t)p * R;
+#pragma unroll
+    for (int i = 0; i < kRecSlots; ++i)
+      if (i < nrec && i < R) rec[i] = rk[i];
+    d.rec_n[p] = (uint8_t)(nrec < 255 ? nrec : 255);
+  }
   t.eval += ev;
 }
 
@@ -403,8 +413,7 @@ __global__ __launch_bounds__(256) void k_core(Dev d) {
     if (k) atomicAdd(&d.cell_ncore[c], k);
     if (p == qs) {
       // first point of its cell: clear the adjacency mask, list heavy cells
-      d.amask[2 * (int64_t)c] = 0ull;
-      d.amask[2 * (int64_t)c + 1] = 0ull;
+      d.amask[c] = 0ull;
       dense = (qe - qs) >= P.minpts;
       if (qe - qs > kHeavy) {
         const long long h = (long long)atomicAdd((unsigned long long*)&d.ctl[C_NHEAVY], 1ull);
@@ -676,7 +685,7 @@ __global__ __launch_bounds__(256) void k_adj(Dev d) {
   // union of the run's finds
   if (act && lane == head) {
     const unsigned long long u = s_m[wv][lane];
-    if (u) atomicOr(&d.amask[2 * (int64_t)c], u);
+    if (u) atomicOr(&d.amask[c], u);
   }
   wave_add(&d.ctl[C_BAND], t.band);
   wave_add(&d.ctl[C_ASYM], t.asym);
@@ -803,12 +812,12 @@ __device__ void adj_heavy(const Dev& d, const Inst& I, const Par& P, int32_t A, 
           const int rm = D == 3 ? kRowMirror3[r] : kRowMirror2[r];
           const int32_t pm = rows_of(d, Bc)[rm];
           const int bm = 5 * rm + (A - (pm & kRowCellMask));
-          if (lane == 0) atomicOr(&d.amask[2 * (int64_t)Bc], 1ull << bm);
+          if (lane == 0) atomicOr(&d.amask[Bc], 1ull << bm);
         }
       }
     }
   }
-  if (lane == 0) d.amask[2 * (int64_t)A] = m0;
+  if (lane == 0) d.amask[A] = m0;
 }
 
 template <bool F32>
@@ -967,7 +976,7 @@ __global__ __launch_bounds__(256) void k_flatten_all(Dev d) {
 // the next pass and exits, so one long chain cannot hold a pass; a wave also
 // exits when its list is empty and no seed source is left.  Every wave
 // reaches the end of the kernel: there is no inter-wave waiting.
-constexpr int kSeen = 256;  // per-wave table of recently recorded collision pairs
+constexpr int kSeen = 256;   // per-wave table of recently recorded collision pairs
 
 __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budget) {
   extern __shared__ int32_t s_lists[];
@@ -997,7 +1006,7 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
   const int64_t seg_end = (int64_t)C * (gw + 1) / W;
   int left = budget;
   int sl = 0;  // seed-list length (wave-uniform)
-  long long pops = 0, ovf = 0, edges = 0;
+  long long pops = 0, ovf = 0, edges = 0, seeds = 0;
   for (;;) {
     if (left <= 0) {
       if (sl > 0) {
@@ -1038,6 +1047,7 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
           ok = __shfl(ok, 0, 64);
           if (ok) {
             if (lane == 0) list[0] = (int32_t)(seg + l);
+            ++seeds;
             sl = 1;
             got = true;
             seg = seg + l + 1;
@@ -1059,35 +1069,34 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
     left -= P;
     pops += P;
     int32_t own = 0;
-    unsigned long long m0 = 0, m1 = 0;
+    unsigned long long m0 = 0;
+    int32_t rA[9];  // the popped cell's near lines, loaded with its label and mask
     if (act) {
       // coherent read: the claim may have been made moments ago on another
       // XCD, whose L2 is not coherent with this one for plain loads
       own = __hip_atomic_load(&d.cell_label[A], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      m0 = d.amask[2 * (int64_t)A];
-      m1 = d.amask[2 * (int64_t)A + 1];
+      m0 = d.amask[A];
     }
-    edges += __popcll(m0) + __popcll(m1);
-    const RowsOf rowsA = rows_of(d, A);
-    while (ballot((m0 | m1) != 0ull)) {
+    {
+      const RowsOf rows = rows_of(d, A);
+#pragma unroll
+      for (int r = 0; r < 9; ++r) rA[r] = (act && m0) ? rows[r] : 0;
+    }
+    edges += __popcll(m0);
+    while (ballot(m0 != 0ull)) {
       constexpr int T = 8;  // neighbours per lane step
       int32_t tg[T];
       bool v[T];
 #pragma unroll
       for (int j = 0; j < T; ++j) {
-        int b = 0;
-        v[j] = true;
-        if (m0) {
-          b = __ffsll((long long)m0) - 1;
-          m0 &= m0 - 1;
-        } else if (m1) {
-          b = 64 + __ffsll((long long)m1) - 1;
-          m1 &= m1 - 1;
-        } else {
-          v[j] = false;
-        }
+        v[j] = m0 != 0ull;
+        const int b = v[j] ? __ffsll((long long)m0) - 1 : 0;
+        m0 &= m0 - 1;
         const int r = (b * 205) >> 10;  // b / 5 for 0 <= b < 128
-        tg[j] = v[j] ? (rowsA[r] & kRowCellMask) + (b - 5 * r) : 0;
+        int32_t first = rA[0];
+#pragma unroll
+        for (int i = 1; i < 9; ++i) first = r == i ? rA[i] : first;
+        tg[j] = v[j] ? (first & kRowCellMask) + (b - 5 * r) : 0;
       }
       // a plain load can only be stale as kUnclaimed (a label is written
       // once), so only those cells pay for a CAS
@@ -1157,6 +1166,8 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
     atomicAdd((unsigned long long*)&d.ctl[C_BUSY_SUM], us);
     atomicMax((unsigned long long*)&d.ctl[C_BUSY_MAX], us);
     atomicAdd((unsigned long long*)&d.ctl[C_POPS], (unsigned long long)pops);
+    atomicMax((unsigned long long*)&d.ctl[C_POPS_MAX], (unsigned long long)pops);
+    atomicMax((unsigned long long*)&d.ctl[C_SEEDS_MAX], (unsigned long long)seeds);
   }
   wave_add(&d.ctl[C_OVF], ovf);
   wave_add(&d.ctl[C_EDGES], edges);

@@ -6,7 +6,7 @@
 //   sort    key[2]    u64 cell keys (ping-pong), val[2] i32 point ids
 //   CSR     cell_start[C+1], cell_key[C], cell_inst[C], rows[C][25] (packed
 //           runs of the 5^(D-1) neighbour lines, centre line first),
-//           cell_ncore[C], cell_label[C] (chain), amask[C][2] (adjacency)
+//           cell_ncore[C], cell_label[C] (chain), amask[C] (near adjacency)
 //   points  (cell-sorted)  X[3][N] FP64 SoA, G[3][N] u32 fixed-point
 //           coordinates (29 fraction bits per cell, wrapping mod 8 cells),
 //           orig[N], cell_of[N], core[N] u8, rec[N][R] + rec_n[N] (the
@@ -29,7 +29,7 @@ constexpr int kRowCntShift = 28;  // packed row: first cell | cell count << 28
 constexpr int32_t kRowCellMask = (1 << kRowCntShift) - 1;
 constexpr int64_t kMaxCells = 1LL << kRowCntShift;
 constexpr int kHeavy = 32;    // cells with more points are "heavy" (wavefront paths)
-constexpr int kMaxRec = 64;   // largest minpts for which neighbour records are kept
+constexpr int kRecSlots = 16;  // neighbour records kept per non-core point (at most)
 
 // Stencil lines, nearest first: (offset along axis 0, offset along axis 1)
 // of line r for D = 3, offset along axis 0 for D = 2.  The mirror of line r
@@ -132,6 +132,8 @@ enum Ctl : int {
   C_ADJ_FREE,   // adjacency bits set without a test (cell inside the eps ball)
   C_FAR_TESTS,  // pair tests of the far-pair pass
   C_FAR_JOINS,  // far pairs that joined two components
+  C_POPS_MAX,   // most cells popped by one expansion wave in one pass
+  C_SEEDS_MAX,  // most chains started by one expansion wave in one pass
   C_NCTL = 40
 };
 
@@ -166,7 +168,7 @@ struct Dev {
   int32_t* cell_inst;   // [N]
   int32_t* cell_ncore;  // [N] core points per cell
   int32_t* cell_label;  // [N] chain owning the cell's core points
-  unsigned long long* amask;  // [N][2] near adjacency: bit 5r+k <-> k-th cell of line r
+  unsigned long long* amask;  // [N] near adjacency: bit 5r+k <-> k-th cell of line r < 9
   int32_t* cell_comp;   // [N] component (root chain) of each core cell, -1 otherwise
   int32_t* heavy;       // heavy cells (more than kHeavy points)
   int32_t* adj_pts;     // core points of light cells (near-adjacency lanes)
@@ -186,7 +188,7 @@ struct Dev {
   uint8_t* core;
   int32_t* rec;         // [N][rec_stride] neighbours of non-core points
   uint8_t* rec_n;       // [N] records written (saturates at 255)
-  int32_t rec_stride;   // 0: no records (minpts > kMaxRec)
+  int32_t rec_stride;   // record slots per point: min(minpts, kRecSlots)
   int32_t* parent;
   int32_t* compmin;
   int2* pairs;          // [waves][pair_per] collision pairs per expansion wave

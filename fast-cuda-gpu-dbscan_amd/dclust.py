@@ -62,7 +62,7 @@ _STAT_I32 = ["hip_error", "key_bits"]
 
 class Stats(C.Structure):
     _fields_ = ([(k, C.c_double) for k in _STAT_F64] + [(k, C.c_int64) for k in _STAT_I64] +
-                [(k, C.c_int32) for k in _STAT_I32] + [("diag", C.c_int64 * 12)])
+                [(k, C.c_int32) for k in _STAT_I32] + [("diag", C.c_int64 * 16)])
 
     def as_dict(self):
         out = {k: getattr(self, k) for k, _ in self._fields_ if k != "diag"}

@@ -115,8 +115,10 @@ typedef struct dclust_stats {
    * lifetime (us), [7] expansion wavefronts, [8] pair tests of the
    * near-adjacency kernel, [9] adjacency bits set without a test, [10]
    * pair tests of the far-pair pass, [11] far pairs that joined two
-   * components */
-  int64_t diag[12];
+   * components, [12] most cells popped by one expansion wave in a pass,
+   * [13] most chains started by one expansion wave in a pass, [14]-[15]
+   * reserved */
+  int64_t diag[16];
 } dclust_stats;
 
 typedef struct dclust_handle dclust_handle;

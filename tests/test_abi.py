@@ -52,4 +52,4 @@ def test_strerror_and_version():
 def test_struct_layout_matches_header():
     assert ctypes.sizeof(dclust.Params) == 40
     assert dclust.Params.eps.offset == 8
-    assert ctypes.sizeof(dclust.Stats) == 9 * 8 + 17 * 8 + 2 * 4 + 12 * 8
+    assert ctypes.sizeof(dclust.Stats) == 9 * 8 + 17 * 8 + 2 * 4 + 16 * 8
