@@ -138,6 +138,86 @@ def cpu_baseline(x, info, budget_s=15.0):
                        f"{dt:.2f} s")}
 
 
+BATCH_DESC = ("cfg4: 4,096 independent 2-D/3-D instances, sizes log-uniform 200-200,000, "
+              "2-32 Gaussian blobs + 5% noise each, eps 0.5-2x the median 10-NN distance, "
+              "minpts in {4,8,16}, FP32 + guard band")
+
+
+def batched_leg(hip, h, steps, warmup, check_oracle=True, count=4096):
+    """One batched clustering of every instance (autotuned launch shape)
+    against one plain single-instance call per instance (library default
+    shape), on the same device-resident inputs; labels compared instance by
+    instance, and a sample checked against the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        insts = list(ex.map(dclust.synth_batch_instance, range(count)))
+    xs = [x for x, _ in insts]
+    infos = [i for _, i in insts]
+    ns = np.array([x.shape[0] for x in xs], dtype=np.int64)
+    dims = np.array([x.shape[1] for x in xs], dtype=np.int64)
+    plain = [dclust.make_params(i["dim"], i["eps"], i["minpts"], precision=dclust.FP32_GUARD)
+             for i in infos]
+    flat = np.concatenate([x.ravel() for x in xs])
+    tot = int(ns.sum())
+    d_x = hip.malloc(flat.nbytes)
+    hip.h2d(d_x, flat)
+    d_b = hip.malloc(tot * 4)
+    d_1 = hip.malloc(tot * 4)
+    t0 = time.perf_counter()
+    tuned = h.autotune_device(d_x, ns, plain)
+    t_tune = time.perf_counter() - t0
+    for _ in range(warmup):
+        h.run_batched_device(d_x, ns, tuned, d_b)
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        errs = h.run_batched_device(d_x, ns, tuned, d_b)
+    hip.sync()
+    t_b = (time.perf_counter() - t0) / steps
+    st_b = h.last_stats
+    off_el = np.concatenate([[0], np.cumsum(ns * dims)])
+    off_pt = np.concatenate([[0], np.cumsum(ns)])
+    h.run_device(d_x, int(ns[0]), plain[0], d_1)  # warm-up
+    hip.sync()
+    t0 = time.perf_counter()
+    for i in range(count):
+        h.run_device(d_x + int(off_el[i]) * 8, int(ns[i]), plain[i], d_1 + int(off_pt[i]) * 4)
+    hip.sync()
+    t_pi = time.perf_counter() - t0
+    lab_b = np.empty(tot, dtype=np.int32)
+    lab_1 = np.empty(tot, dtype=np.int32)
+    hip.d2h(lab_b, d_b)
+    hip.d2h(lab_1, d_1)
+    bad = [i for i in range(count)
+           if not np.array_equal(lab_b[off_pt[i]:off_pt[i + 1]], lab_1[off_pt[i]:off_pt[i + 1]])]
+    out = {"workload": BATCH_DESC, "instances": count, "points": tot,
+           "batched_ms": round(t_b * 1e3, 3), "per_instance_ms": round(t_pi * 1e3, 3),
+           "speedup_batched_vs_per_instance": round(t_pi / t_b, 2),
+           "batched_mpoints_s": round(tot / t_b / 1e6, 2),
+           "per_instance_mpoints_s": round(tot / t_pi / 1e6, 2),
+           "instance_errors": int(np.count_nonzero(errs)),
+           "instances_differing_batched_vs_per_instance": len(bad),
+           "tuned_shape": {"seed_list_size": tuned[0].seed_list_size,
+                           "blocks": tuned[0].blocks, "block_threads": tuned[0].block_threads},
+           "autotune_s": round(t_tune, 2),
+           "phases_ms": {k: round(st_b[k], 3) for k in st_b if k.startswith("t_")}}
+    if check_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle  # checker only, untimed
+
+        order = np.argsort(ns)
+        sample = sorted(set(order[np.linspace(0, count - 1, 24).astype(int)].tolist()))
+        mism = 0
+        for i in sample:
+            ref = oracle.fast_dbscan(xs[i], infos[i]["eps"], infos[i]["minpts"])[0]
+            mism += int(np.sum(ref != lab_b[off_pt[i]:off_pt[i + 1]]))
+        out["oracle_sample"] = {"instances": len(sample), "label_mismatches": mism}
+    for ptr in (d_x, d_b, d_1):
+        hip.free(ptr)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +228,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-vs-per-instance leg (cfg4)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a plain copy of known size, "
                          "used to calibrate the HBM byte counters of a profiler run)")
@@ -246,6 +328,9 @@ def main():
         out["parity"] = {"label_mismatches": int(np.sum(lab != ref)),
                          "core_mismatches": int(np.sum(core != rcore)),
                          "oracle_asym_pairs": int(asym)}
+    if not args.no_batched:
+        out["batched"] = batched_leg(hip, h, max(2, args.steps // 2), 1,
+                                     check_oracle=not args.no_parity)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(x, info)
     h.close()

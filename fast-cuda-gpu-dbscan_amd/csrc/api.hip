@@ -15,6 +15,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "dclust.h"
@@ -51,6 +53,9 @@ struct dclust_handle {
   const int32_t* last_orig = nullptr;  // device, cell-sorted input indices
   uint8_t* last_core_out = nullptr;    // device, input-order staging
   int last_err = 0;
+  // launch shapes found by dclust_autotune_device, keyed by (dimension mix,
+  // log2 of the total point count, precision form)
+  std::map<std::tuple<int, int, int>, dclust_params> tuned;
 
   template <class T>
   T* get(int slot, size_t count, hipError_t* err) {
@@ -86,7 +91,7 @@ enum Slot {
   S_DTOT, S_PASS, S_CSTART, S_CKEY, S_CINST, S_CM, S_ROWS, S_CELLOF, S_X0, S_X1, S_X2,
   S_G0, S_G1, S_G2, S_ORIG, S_CORE, S_LABEL, S_PARENT, S_COMPMIN, S_PAIRS, S_Q0, S_Q1,
   S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT, S_PAIRCNT, S_SEGPOS, S_LINES, S_AMASK,
-  S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP, S_ADJPTS, S_CORECELLS
+  S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP, S_ADJPTS, S_CORECELLS, S_TUNELAB
 };
 
 // line table entries (8 bytes each) above which neighbour lines are found
@@ -590,9 +595,87 @@ int dclust_last_core_flags(dclust_handle* h, uint8_t* core_out, int64_t n) {
   return DCLUST_OK;
 }
 
+// Tuning-table key of a problem: dimension mix (2, 3, or 5 for both), size
+// class, precision form.
+static std::tuple<int, int, int> tune_key(int32_t count, const int64_t* n,
+                                          const dclust_params* params) {
+  int dims = 0;
+  int64_t tot = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    dims |= params[i].dim == 3 ? 2 : 1;
+    tot += n[i];
+  }
+  int lg = 0;
+  while ((1LL << (lg + 1)) <= tot) ++lg;
+  const int dm = dims == 3 ? 5 : (dims == 2 ? 3 : 2);
+  return std::make_tuple(dm, lg, params[0].precision);
+}
+
+int dclust_autotune_device(dclust_handle* h, int32_t count, const double* d_xyz, const int64_t* n,
+                           dclust_params* params) {
+  if (!h || count <= 0 || !d_xyz || !n || !params) return DCLUST_E_PARAM;
+  if (hipSetDevice(h->device) != hipSuccess) return DCLUST_E_HIP;
+  int64_t tot = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    int c = check_params(&params[i]);
+    if (c) return c;
+    if (n[i] < 0) return DCLUST_E_PARAM;
+    tot += n[i];
+  }
+  hipError_t herr = hipSuccess;
+  int32_t* lab = h->get<int32_t>(S_TUNELAB, std::max<int64_t>(tot, 1), &herr);
+  if (!lab) return DCLUST_E_OOM;
+  // candidates: seed-list length x workgroup size x workgroups per CU; each
+  // is run twice and scored by its faster run
+  const int seeds_c[] = {256, 512, 1024, 2048};
+  const int threads_c[] = {128, 256};
+  const int per_cu_c[] = {4, 8, 16};
+  std::vector<dclust_params> trial(params, params + count);
+  double best = 1e30;
+  dclust_params best_p = params[0];
+  for (int sd : seeds_c)
+    for (int th : threads_c)
+      for (int pc : per_cu_c) {
+        const int waves_per_cu = pc * th / 64;
+        if (waves_per_cu > 32) continue;  // beyond what a CU holds
+        for (auto& q : trial) {
+          q.seed_list_size = sd;
+          q.block_threads = th;
+          q.blocks = h->num_cu * pc;
+          q.extra_collision = 0;
+        }
+        double t = 1e30;
+        for (int rep = 0; rep < 2; ++rep) {
+          dclust_stats st;
+          int rc = run_device(h, count, d_xyz, n, trial.data(), lab, nullptr, &st);
+          if (rc != DCLUST_OK) return rc;
+          t = std::min(t, st.t_total_ms);
+        }
+        if (t < best) {
+          best = t;
+          best_p = trial[0];
+        }
+      }
+  for (int32_t i = 0; i < count; ++i) {
+    params[i].seed_list_size = best_p.seed_list_size;
+    params[i].block_threads = best_p.block_threads;
+    params[i].blocks = best_p.blocks;
+    params[i].extra_collision = best_p.extra_collision;
+  }
+  h->tuned[tune_key(count, n, params)] = best_p;
+  return DCLUST_OK;
+}
+
 int dclust_autotune(dclust_handle* h, int64_t n, dclust_params* p) {
   if (!h || !p) return DCLUST_E_PARAM;
   bool f32 = p->precision == DCLUST_PRECISION_FP32_GUARD;
+  auto it = h->tuned.find(tune_key(1, &n, p));
+  if (it != h->tuned.end()) {
+    if (p->seed_list_size == 0) p->seed_list_size = it->second.seed_list_size;
+    if (p->block_threads == 0) p->block_threads = it->second.block_threads;
+    if (p->blocks == 0) p->blocks = it->second.blocks;
+    return DCLUST_OK;
+  }
   RunShape r = pick_shape(h, *p, n, f32);
   if (p->seed_list_size == 0) p->seed_list_size = r.seeds;
   if (p->block_threads == 0) p->block_threads = r.threads;

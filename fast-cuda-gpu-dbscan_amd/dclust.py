@@ -103,6 +103,7 @@ def _lib():
                                                 C.POINTER(Stats)]
         L.dclust_last_core_flags.argtypes = [_P, _P, C.c_int64]
         L.dclust_autotune.argtypes = [_P, C.c_int64, C.POINTER(Params)]
+        L.dclust_autotune_device.argtypes = [_P, C.c_int32, _P, _P, _P]
         L.dclust_import_dataset.argtypes = [C.c_char_p, C.c_int32, C.c_int64, _P,
                                             C.POINTER(C.c_int64)]
         L.dclust_write_labels.argtypes = [C.c_char_p, _P, C.c_int64]
@@ -256,6 +257,16 @@ class Handle:
     def autotune(self, n: int, params: Params) -> Params:
         _check(_lib().dclust_autotune(self._h, int(n), C.byref(params)), "dclust_autotune")
         return params
+
+    def autotune_device(self, d_xyz: int, ns, params_list):
+        """Measure the candidate launch shapes on device-resident instances
+        and return the params with the fastest shape filled in."""
+        k = len(ns)
+        ns = np.ascontiguousarray(ns, dtype=np.int64)
+        parr = (Params * k)(*params_list)
+        _check(_lib().dclust_autotune_device(self._h, k, C.c_void_p(d_xyz), ns.ctypes.data, parr),
+               "dclust_autotune_device")
+        return list(parr)
 
 
 def import_dataset(path: str, dim: int, max_points: int = 0):

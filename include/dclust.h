@@ -169,9 +169,22 @@ int dclust_run_batched_device(dclust_handle* h, int32_t count,
  * host; n must equal the last run's total point count. */
 int dclust_last_core_flags(dclust_handle* h, uint8_t* core_out, int64_t n);
 
-/* Autotuned launch shape for a problem (seed list, blocks, threads); fills
- * the zero fields of *p from the library's tuning table. */
+/* Launch shape for a problem of n points (seed list, blocks, threads,
+ * collision buffer): fills the zero fields of *p from the shapes that
+ * dclust_autotune_device has measured on this handle for the same dimension,
+ * size class (log2 n) and precision, else from the library's rule. */
 int dclust_autotune(dclust_handle* h, int64_t n, dclust_params* p);
+
+/* Empirical autotuner (no counterpart in the reference, whose launch shape is
+ * fixed at compile time, common.h:6-25): clusters the given device-resident
+ * instances (layout of dclust_run_batched_device; count = 1 for one dataset)
+ * with every candidate shape -- seed-list length {256, 512, 1024, 2048} x
+ * workgroup size {128, 256} x workgroups per CU {4, 8, 16} -- keeps the
+ * fastest, writes it into params[0..count) and remembers it for
+ * dclust_autotune.  The grid does not depend on PARTITION_SIZE (the cell
+ * width is fixed by eps), so r is not a tuning dimension here. */
+int dclust_autotune_device(dclust_handle* h, int32_t count, const double* d_xyz,
+                           const int64_t* n, dclust_params* params);
 
 /* Reads a CSV dataset with the grammar of ImportDataset (main.cu:469-498):
  * lines of comma-separated numbers, all values flattened row-major, dim
