@@ -91,7 +91,7 @@ enum Slot {
   S_DTOT, S_PASS, S_CSTART, S_CKEY, S_CINST, S_CM, S_ROWS, S_CELLOF, S_X0, S_X1, S_X2,
   S_G0, S_G1, S_G2, S_ORIG, S_CORE, S_LABEL, S_PARENT, S_COMPMIN, S_PAIRS, S_Q0, S_Q1,
   S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT, S_PAIRCNT, S_SEGPOS, S_LINES, S_AMASK,
-  S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP, S_ADJPTS, S_CORECELLS, S_TUNELAB
+  S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP, S_ADJPTS, S_CORECELLS, S_TUNELAB, S_STAT
 };
 
 // line table entries (8 bytes each) above which neighbour lines are found
@@ -252,6 +252,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     d.chunks = h->get<Chunk>(S_CHUNK, nchunks, &E);
     d.bbox = h->get<unsigned long long>(S_BBOX, (size_t)ninst * 8, &E);
     d.ctl = h->get<long long>(S_CTL, C_NCTL, &E);
+    d.stat = h->get<long long>(S_STAT, (size_t)C_NCTL * kStripes, &E);
     d.key[0] = h->get<uint64_t>(S_KEY0, ntot, &E);
     d.key[1] = h->get<uint64_t>(S_KEY1, ntot, &E);
     d.val[0] = h->get<int32_t>(S_VAL0, ntot, &E);
@@ -264,8 +265,15 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     d.cell_key = h->get<uint64_t>(S_CKEY, ntot, &E);
     d.cell_xyz = h->get<uint64_t>(S_CXYZ, ntot, &E);
     d.cell_comp = h->get<int32_t>(S_CCOMP, ntot, &E);
-    d.adj_pts = h->get<int32_t>(S_ADJPTS, ntot, &E);
-    d.core_cells = h->get<int32_t>(S_CORECELLS, ntot, &E);
+    // striped work lists: sub-list capacity = the most its workgroups append
+    const int64_t cap_p = (point_blocks(ntot) + kStripes - 1) / kStripes * 256;
+    const int64_t cap_c = (cell_comp_blocks(ntot) + kStripes - 1) / kStripes * 1024;
+    d.adj_pts = SList{h->get<int32_t>(S_ADJPTS, (size_t)cap_p * kStripes, &E),
+                      (unsigned long long*)(d.stat + (int64_t)C_NADJPTS * kStripes), cap_p};
+    d.heavy = SList{h->get<int32_t>(S_HEAVY, (size_t)cap_p * kStripes, &E),
+                    (unsigned long long*)(d.stat + (int64_t)C_NHEAVY * kStripes), cap_p};
+    d.core_cells = SList{h->get<int32_t>(S_CORECELLS, (size_t)cap_c * kStripes, &E),
+                         (unsigned long long*)(d.stat + (int64_t)C_NCORECELLS * kStripes), cap_c};
     d.cell_inst = h->get<int32_t>(S_CINST, ntot, &E);
     d.cell_ncore = h->get<int32_t>(S_CM, ntot, &E);
     d.rows = h->get<int32_t>(S_ROWS, (size_t)ntot * kMaxRows, &E);
@@ -280,7 +288,6 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     d.core = h->get<uint8_t>(S_CORE, ntot, &E);
     d.cell_label = h->get<int32_t>(S_LABEL, ntot, &E);
     d.amask = h->get<unsigned long long>(S_AMASK, (size_t)ntot, &E);
-    d.heavy = h->get<int32_t>(S_HEAVY, ntot / (kHeavy + 1) + 1, &E);
     d.rec_n = h->get<uint8_t>(S_RECN, ntot, &E);
     if (rec_stride > 0) d.rec = h->get<int32_t>(S_REC, (size_t)ntot * rec_stride, &E);
     d.parent = h->get<int32_t>(S_PARENT, ntot, &E);
@@ -309,6 +316,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
                           hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(h->ev[E_START], s));
     launch_fill_i64(d.ctl, C_NCTL, 0, s);
+    launch_fill_i64(d.stat, (int64_t)C_NCTL * kStripes, 0, s);
     launch_fill_i32(d.pass_scatter, kMaxPasses, 0, s);
     launch_bbox_init(d, s);
     launch_index_head(d, nchunks, s);
@@ -373,8 +381,13 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     launch_relabel(d, ntot, s);
     HIPCHK(hipEventRecord(h->ev[E_END], s));
     HIPCHK(hipGetLastError());
+    std::vector<long long> stat_h((size_t)C_NCTL * kStripes);
     HIPCHK(hipMemcpyAsync(ctl_h, d.ctl, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(stat_h.data(), d.stat, sizeof(long long) * stat_h.size(),
+                          hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    for (int c = 0; c < C_NCTL; ++c)
+      for (int k = 0; k < kStripes; ++k) ctl_h[c] += stat_h[(size_t)c * kStripes + k];
     t_pass0_ms = ev_ms(h, E_ADJ, E_PASS0);
     h->last_n = ntot;
     h->last_core = d.core;

@@ -249,11 +249,10 @@ __device__ __forceinline__ void mask_set(unsigned long long& m0, unsigned long l
     }                                              \
   } while (0)
 
-// Appends `value` of the threads with `take` to `list`, in thread order,
-// with one atomic on `counter` per workgroup (a counter shared by a whole
-// launch is the contended address).  Every thread of the workgroup calls it.
-__device__ __forceinline__ void block_append(bool take, int32_t value, int32_t* list,
-                                             long long* counter) {
+// Appends `value` of the threads with `take` to the workgroup's sub-list of
+// L, in thread order, with one atomic per workgroup.  Every thread of the
+// workgroup calls it.
+__device__ __forceinline__ void block_append(bool take, int32_t value, const SList& L) {
   __shared__ int s_cnt[16];
   __shared__ long long s_base;
   const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -263,12 +262,14 @@ __device__ __forceinline__ void block_append(bool take, int32_t value, int32_t* 
   if (threadIdx.x == 0) {
     int tot = 0;
     for (int w = 0; w < nw; ++w) tot += s_cnt[w];
-    s_base = tot ? (long long)atomicAdd((unsigned long long*)counter, (unsigned long long)tot) : 0;
+    s_base = tot ? (long long)atomicAdd(&L.cnt[blockIdx.x & (kStripes - 1)],
+                                        (unsigned long long)tot)
+                 : 0;
   }
   __syncthreads();
   long long base = s_base;
   for (int w = 0; w < wv; ++w) base += s_cnt[w];
-  if (take) list[base + mbcnt(m)] = value;
+  if (take) L.data[(int64_t)(blockIdx.x & (kStripes - 1)) * L.cap + base + mbcnt(m)] = value;
   __syncthreads();  // s_cnt / s_base are reused by the next call
 }
 
@@ -400,7 +401,7 @@ __global__ __launch_bounds__(256) void k_core(Dev d) {
                 (core_lane<2, F32>(d, P, p, c, qs, qe, R, t, isc)));
   if (act) d.core[p] = isc;
   // core points of light cells: the lanes of the near-adjacency pass
-  block_append(isc && qe - qs <= kHeavy, p, d.adj_pts, &d.ctl[C_NADJPTS]);
+  block_append(isc && qe - qs <= kHeavy, p, d.adj_pts);
   // per-cell core counts: the lanes of one cell form a contiguous run
   const bool head = act && (p == qs || lane == 0);
   const unsigned long long mh = ballot(head), mc = ballot(isc);
@@ -412,21 +413,19 @@ __global__ __launch_bounds__(256) void k_core(Dev d) {
     const int k = __popcll(mc & seg);
     if (k) atomicAdd(&d.cell_ncore[c], k);
     if (p == qs) {
-      // first point of its cell: clear the adjacency mask, list heavy cells
+      // first point of its cell: clear the adjacency mask
       d.amask[c] = 0ull;
       dense = (qe - qs) >= P.minpts;
-      if (qe - qs > kHeavy) {
-        const long long h = (long long)atomicAdd((unsigned long long*)&d.ctl[C_NHEAVY], 1ull);
-        d.heavy[h] = c;
-      }
     }
   }
-  wave_add(&d.ctl[C_NCORE], isc ? 1 : 0);
-  wave_add(&d.ctl[C_DENSE], dense);
-  wave_add(&d.ctl[C_EVAL], t.eval);
-  wave_add(&d.ctl[C_BAND_CORE], t.band);
-  wave_add(&d.ctl[C_BAND], t.band);
-  wave_add(&d.ctl[C_FLIP], t.flip);
+  // heavy cells, listed once (by their first point)
+  block_append(act && p == qs && qe - qs > kHeavy, c, d.heavy);
+  stat_add(d, C_NCORE, isc ? 1 : 0);
+  stat_add(d, C_DENSE, dense);
+  stat_add(d, C_EVAL, t.eval);
+  stat_add(d, C_BAND_CORE, t.band);
+  stat_add(d, C_BAND, t.band);
+  stat_add(d, C_FLIP, t.flip);
 }
 
 // ------------------------------------------------------------- adjacency --
@@ -545,7 +544,7 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
                          bool act, int head, int rank, unsigned long long* xm,
                          unsigned long long& m0, Tally& t, long long& freeb) {
   constexpr int NN = Near<D>::lines;
-  constexpr int B = Batch<F32>::value;
+  constexpr int B = 4;  // near cells need few tests: a short batch keeps registers low
   constexpr uint32_t kFrac = (1u << kFixBits) - 1u;
   uint32_t pg[D];
   double px[D];
@@ -653,10 +652,11 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
 template <bool F32>
 __global__ __launch_bounds__(256) void k_adj(Dev d) {
   __shared__ unsigned long long s_m[4][64];
+  __shared__ long long s_pre[kStripes + 1];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
-  const bool act = i < d.ctl[C_NADJPTS];
-  const int32_t p = act ? d.adj_pts[i] : 0;
+  const bool act = i < slist_prefix(d.adj_pts, s_pre);
+  const int32_t p = act ? slist_at(d.adj_pts, s_pre, i) : 0;
   const int32_t c = act ? d.cell_of[p] : -1;
   // the lanes of one cell form a contiguous run; head = its first lane here
   const int32_t cprev = __shfl_up(c, 1, 64);
@@ -687,10 +687,10 @@ __global__ __launch_bounds__(256) void k_adj(Dev d) {
     const unsigned long long u = s_m[wv][lane];
     if (u) atomicOr(&d.amask[c], u);
   }
-  wave_add(&d.ctl[C_BAND], t.band);
-  wave_add(&d.ctl[C_ASYM], t.asym);
-  wave_add(&d.ctl[C_ADJ_TESTS], t.eval);
-  wave_add(&d.ctl[C_ADJ_FREE], freeb);
+  stat_add(d, C_BAND, t.band);
+  stat_add(d, C_ASYM, t.asym);
+  stat_add(d, C_ADJ_TESTS, t.eval);
+  stat_add(d, C_ADJ_FREE, freeb);
 }
 
 // Heavy cell A (more than kHeavy points), one wavefront, near cells only:
@@ -825,10 +825,11 @@ __global__ __launch_bounds__(64) void k_adj_heavy(Dev d) {
   __shared__ int32_t s_qi[kQCache];
   __shared__ uint32_t s_qg[kQCache * 3];
   __shared__ double s_qx[F32 ? 1 : kQCache * 3];
-  const long long nh = d.ctl[C_NHEAVY];
+  __shared__ long long s_pre[kStripes + 1];
+  const long long nh = slist_prefix(d.heavy, s_pre);
   Tally t;
   for (long long h = blockIdx.x; h < nh; h += gridDim.x) {
-    const int32_t A = d.heavy[h];
+    const int32_t A = slist_at(d.heavy, s_pre, h);
     const Inst& I = *inst_of_cell(d, A);
     const Par P(I);
     if (I.dim == 3)
@@ -837,8 +838,8 @@ __global__ __launch_bounds__(64) void k_adj_heavy(Dev d) {
       adj_heavy<2, F32>(d, I, P, A, s_qi, s_qg, s_qx, t);
     wave_sync();  // the query cache is reused
   }
-  wave_add(&d.ctl[C_BAND], t.band);
-  wave_add(&d.ctl[C_ASYM], t.asym);
+  stat_add(d, C_BAND, t.band);
+  stat_add(d, C_ASYM, t.asym);
 }
 
 // ------------------------------------------------------------ far pairs --
@@ -846,9 +847,8 @@ __global__ __launch_bounds__(64) void k_adj_heavy(Dev d) {
 // its chain), and the compacted list of core cells.
 __global__ __launch_bounds__(1024) void k_cell_comp(Dev d) {
   const int64_t C = d.ctl[C_NCELL];
-  for (int64_t c0 = blockIdx.x * (int64_t)blockDim.x; c0 < C;
-       c0 += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t c = c0 + threadIdx.x;
+  {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = c < C;
     const bool hc = in && d.cell_ncore[c] > 0;
     int32_t chain = -1, cmin = kNoKey;
@@ -875,7 +875,7 @@ __global__ __launch_bounds__(1024) void k_cell_comp(Dev d) {
       pend = pend && !mine;
       m = ballot(pend);
     }
-    block_append(hc, (int32_t)c, d.core_cells, &d.ctl[C_NCORECELLS]);
+    block_append(hc, (int32_t)c, d.core_cells);
   }
 }
 
@@ -931,9 +931,10 @@ __device__ long long far_cell(const Dev& d, const Inst& I, const Par& P, int32_t
 
 template <bool F32>
 __global__ __launch_bounds__(256) void k_far(Dev d) {
+  __shared__ long long s_pre[kStripes + 1];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool act = i < d.ctl[C_NCORECELLS];
-  const int32_t A = act ? d.core_cells[i] : 0;
+  const bool act = i < slist_prefix(d.core_cells, s_pre);
+  const int32_t A = act ? slist_at(d.core_cells, s_pre, i) : 0;
   int dim = 0;
   const Inst* I = d.inst;
   if (act) {
@@ -945,10 +946,10 @@ __global__ __launch_bounds__(256) void k_far(Dev d) {
   const Par P(*I);
   DCLUST_BY_DIM(act, dim, (joins = far_cell<3, F32>(d, *I, P, A, t, freeb)),
                 (joins = far_cell<2, F32>(d, *I, P, A, t, freeb)));
-  wave_add(&d.ctl[C_BAND], t.band);
-  wave_add(&d.ctl[C_ASYM], t.asym);
-  wave_add(&d.ctl[C_FAR_TESTS], t.eval);
-  wave_add(&d.ctl[C_FAR_JOINS], joins);
+  stat_add(d, C_BAND, t.band);
+  stat_add(d, C_ASYM, t.asym);
+  stat_add(d, C_FAR_TESTS, t.eval);
+  stat_add(d, C_FAR_JOINS, joins);
 }
 
 // Roots of every chain after the far-pair unions, and the minimum core
@@ -1169,8 +1170,8 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
     atomicMax((unsigned long long*)&d.ctl[C_POPS_MAX], (unsigned long long)pops);
     atomicMax((unsigned long long*)&d.ctl[C_SEEDS_MAX], (unsigned long long)seeds);
   }
-  wave_add(&d.ctl[C_OVF], ovf);
-  wave_add(&d.ctl[C_EDGES], edges);
+  stat_add(d, C_OVF, ovf);
+  stat_add(d, C_EDGES, edges);
 }
 
 // Resets the counters of the queue that pass `pass` fills and of the head
@@ -1432,9 +1433,9 @@ __global__ __launch_bounds__(256) void k_border(Dev d) {
   DCLUST_BY_DIM(act, dim, (best = border_lane<3, F32>(d, P, p, c, qs, qe, R, t, ovf)),
                 (best = border_lane<2, F32>(d, P, p, c, qs, qe, R, t, ovf)));
   if (act) d.keyv[p] = best;
-  wave_add(&d.ctl[C_BAND], t.band);
-  wave_add(&d.ctl[C_FLIP], t.flip);
-  wave_add(&d.ctl[C_RECOVF], ovf);
+  stat_add(d, C_BAND, t.band);
+  stat_add(d, C_FLIP, t.flip);
+  stat_add(d, C_RECOVF, ovf);
 }
 
 // ------------------------------------------------------------ rank + out --
@@ -1497,7 +1498,7 @@ __global__ __launch_bounds__(256) void k_output(Dev d) {
       noise += lab == 0;
     }
   }
-  wave_add(&d.ctl[C_NNOISE], noise);
+  stat_add(d, C_NNOISE, noise);
 }
 
 // ------------------------------------------------------------- launchers --
@@ -1507,6 +1508,9 @@ static int point_grid(int64_t n) {
   b = (b + 7) / 8 * 8;
   return (int)(b < 8 ? 8 : b);
 }
+
+int64_t point_blocks(int64_t n) { return point_grid(n); }
+int64_t cell_comp_blocks(int64_t n) { return (n + 1023) / 1024 > 0 ? (n + 1023) / 1024 : 1; }
 
 void launch_core(const Dev& d, bool f32, hipStream_t s) {
   if (f32)
@@ -1558,7 +1562,7 @@ void launch_merge(const Dev& d, int grid, hipStream_t s) {
 }
 
 void launch_far(const Dev& d, bool f32, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_cell_comp, dim3(grid / 4), dim3(1024), 0, s, d);
+  hipLaunchKernelGGL(k_cell_comp, dim3(cell_comp_blocks(d.n)), dim3(1024), 0, s, d);
   if (f32)
     hipLaunchKernelGGL(k_far<true>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
   else

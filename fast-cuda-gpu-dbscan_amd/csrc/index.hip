@@ -755,8 +755,8 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
 #pragma unroll
     for (int j = 0; j < 5; ++j) d.rows[(int64_t)(5 * g + j) * d.rstride + c] = packed[j];
   }
-  wave_add(&d.ctl[C_CAND], cand);
-  wave_add(&d.ctl[C_NBRC], nbrc);
+  stat_add(d, C_CAND, cand);
+  stat_add(d, C_NBRC, nbrc);
 }
 
 // ------------------------------------------------------------- launchers --

@@ -137,6 +137,20 @@ enum Ctl : int {
   C_NCTL = 40
 };
 
+// Statistics counters are striped: a wave adds to one of kStripes copies.
+constexpr int kStripes = 64;
+
+// A work list filled by many workgroups: kStripes sub-lists (workgroup b
+// appends to sub-list b % kStripes, with its own counter), so that the
+// appends do not serialise on one address.  Element i of the whole list is
+// found through the prefix of the sub-list counts.  The counters live in
+// the striped statistics array, so the host reads the total like a counter.
+struct SList {
+  int32_t* data;             // [kStripes][cap]
+  unsigned long long* cnt;   // [kStripes]
+  int64_t cap;
+};
+
 // Sort-pass bookkeeping: pass p is active when p*8 < key bits; its scatter
 // runs unless every key has the same digit (then the data is already in
 // order and the buffers are not swapped).
@@ -152,6 +166,7 @@ struct Dev {
   const Chunk* chunks;
   unsigned long long* bbox;  // [ninst][6] ordered-u64 min0..2 max0..2
   long long* ctl;
+  long long* stat;  // [C_NCTL][kStripes] statistics counters (summed by the host)
 
   // radix sort
   uint64_t* key[2];
@@ -170,9 +185,9 @@ struct Dev {
   int32_t* cell_label;  // [N] chain owning the cell's core points
   unsigned long long* amask;  // [N] near adjacency: bit 5r+k <-> k-th cell of line r < 9
   int32_t* cell_comp;   // [N] component (root chain) of each core cell, -1 otherwise
-  int32_t* heavy;       // heavy cells (more than kHeavy points)
-  int32_t* adj_pts;     // core points of light cells (near-adjacency lanes)
-  int32_t* core_cells;  // cells with core points (far-pair lanes)
+  SList heavy;          // heavy cells (more than kHeavy points)
+  SList adj_pts;        // core points of light cells (near-adjacency lanes)
+  SList core_cells;     // cells with core points (far-pair lanes)
   int32_t* rows;        // [kMaxRows][rstride] packed neighbour lines (first cell, count),
                         // line-major so that consecutive cells are adjacent
   int64_t rstride;      // >= number of cells
@@ -282,6 +297,41 @@ __device__ __forceinline__ T wave_incl_scan(T v) {
 __device__ __forceinline__ void wave_add(long long* p, long long v) {
   long long s = wave_sum(v);
   if (lane_id() == 0 && s != 0) atomicAdd((unsigned long long*)p, (unsigned long long)s);
+}
+
+// Statistics counter update: the wave's sum goes to one of kStripes copies
+// of the counter, picked by the wave's position in the launch, so that a
+// launch of many waves does not serialise on one address.  The host sums
+// the copies.
+__device__ __forceinline__ void stat_add(const Dev& d, int ctr, long long v) {
+  const long long s = wave_sum(v);
+  if (lane_id() == 0 && s != 0) {
+    const unsigned w = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kStripes - 1);
+    atomicAdd((unsigned long long*)&d.stat[ctr * kStripes + w], (unsigned long long)s);
+  }
+}
+
+// Prefix of the sub-list counts of L in LDS (s_pre[kStripes + 1]); returns
+// the list length.  Every thread of the workgroup calls it.
+__device__ __forceinline__ long long slist_prefix(const SList& L, long long* s_pre) {
+  if (threadIdx.x < kStripes) {
+    const long long v = (long long)L.cnt[threadIdx.x];
+    const long long incl = wave_incl_scan(v);
+    s_pre[threadIdx.x + 1] = incl;
+    if (threadIdx.x == 0) s_pre[0] = 0;
+  }
+  __syncthreads();
+  return s_pre[kStripes];
+}
+
+// Element i (< length) of the list.
+__device__ __forceinline__ int32_t slist_at(const SList& L, const long long* s_pre, long long i) {
+  int lo = 0, hi = kStripes - 1;  // last sub-list whose range starts at or before i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s_pre[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return L.data[(int64_t)lo * L.cap + (i - s_pre[lo])];
 }
 
 // Monotone map double -> u64 (for atomic min/max of doubles).

@@ -30,6 +30,9 @@ void launch_seg_init(const Dev& d, int64_t waves, hipStream_t s);
 void launch_expand(const Dev& d, int pass, int grid, int seeds, int threads, int budget,
                    hipStream_t s);
 void launch_merge(const Dev& d, int grid, hipStream_t s);
+// workgroups of the producers of the striped work lists (their capacity)
+int64_t point_blocks(int64_t n);
+int64_t cell_comp_blocks(int64_t n);
 // far stencil pairs across components, then final roots
 void launch_far(const Dev& d, bool f32, int grid, hipStream_t s);
 void launch_border(const Dev& d, bool f32, hipStream_t s);
