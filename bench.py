@@ -328,6 +328,11 @@ def main():
         out["parity"] = {"label_mismatches": int(np.sum(lab != ref)),
                          "core_mismatches": int(np.sum(core != rcore)),
                          "oracle_asym_pairs": int(asym)}
+    # the same clustering from host arrays (H2D + D2H included), one run
+    t0 = time.perf_counter()
+    h.run(x, info["eps"], info["minpts"], r=info["partitions_r"], seeds=info["seed_list_size"],
+          precision=prec)
+    out["pcie_inclusive_mpoints_s"] = round(n / (time.perf_counter() - t0) / 1e6, 2)
     if not args.no_batched:
         out["batched"] = batched_leg(hip, h, max(2, args.steps // 2), 1,
                                      check_oracle=not args.no_parity)
