@@ -28,3 +28,49 @@ def golden():
 
     with np.load(os.path.join(ROOT, "tests", "golden", "golden.npz"), allow_pickle=False) as z:
         return {k: z[k] for k in z.files}
+
+
+class DeviceMem:
+    """Device buffers through the HIP runtime (plumbing for the *_device
+    entry points)."""
+
+    def __init__(self):
+        import ctypes as C
+
+        self.C = C
+        self.L = C.CDLL("libamdhip64.so")
+        self.L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.L.hipFree.argtypes = [C.c_void_p]
+        self.L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.live = []
+
+    def upload(self, arr):
+        C = self.C
+        p = C.c_void_p()
+        assert self.L.hipMalloc(C.byref(p), max(arr.nbytes, 1)) == 0
+        assert self.L.hipMemcpy(p, arr.ctypes.data, arr.nbytes, 1) == 0
+        self.live.append(p.value)
+        return p.value
+
+    def alloc(self, nbytes):
+        C = self.C
+        p = C.c_void_p()
+        assert self.L.hipMalloc(C.byref(p), max(nbytes, 1)) == 0
+        self.live.append(p.value)
+        return p.value
+
+    def download(self, arr, ptr):
+        assert self.L.hipMemcpy(arr.ctypes.data, self.C.c_void_p(ptr), arr.nbytes, 2) == 0
+        return arr
+
+    def free_all(self):
+        for p in self.live:
+            self.L.hipFree(self.C.c_void_p(p))
+        self.live = []
+
+
+@pytest.fixture
+def devmem():
+    m = DeviceMem()
+    yield m
+    m.free_all()

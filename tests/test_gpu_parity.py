@@ -159,3 +159,58 @@ def test_heavy_cells(handle, prec, dim, minpts):
     x = np.concatenate(parts)
     st = _check(handle, x, eps, minpts, prec)
     assert st["diag"][2] > 0  # heavy cells present
+
+
+def _batch(limit=60, max_n=40000):
+    xs, params = [], []
+    for i in range(limit):
+        x, info = dclust.synth_batch_instance(i)
+        if x.shape[0] > max_n:
+            continue
+        xs.append(x)
+        params.append(dclust.make_params(x.shape[1], info["eps"], info["minpts"],
+                                         precision=dclust.FP32_GUARD))
+    return xs, params
+
+
+def test_batched_equals_per_instance_and_autotune(handle, devmem):
+    xs, params = _batch()
+    assert len({x.shape[1] for x in xs}) == 2  # 2-D and 3-D instances in one batch
+    ns = np.array([x.shape[0] for x in xs], dtype=np.int64)
+    flat = np.concatenate([x.ravel() for x in xs])
+    d_x = devmem.upload(flat)
+    d_lab = devmem.alloc(int(ns.sum()) * 4)
+    tuned = handle.autotune_device(d_x, ns, params)
+    shape = (tuned[0].seed_list_size, tuned[0].blocks, tuned[0].block_threads)
+    assert shape[0] in (256, 512, 1024, 2048) and shape[2] in (128, 256) and shape[1] > 0
+    assert all((t.seed_list_size, t.blocks, t.block_threads) == shape for t in tuned)
+    errs = handle.run_batched_device(d_x, ns, tuned, d_lab)
+    assert not np.any(errs)
+    lab = devmem.download(np.empty(int(ns.sum()), dtype=np.int32), d_lab)
+    off = np.concatenate([[0], np.cumsum(ns)])
+    for i, (x, p) in enumerate(zip(xs, params)):
+        single = handle.run(x, p.eps, p.minpts, precision=dclust.FP32_GUARD)
+        assert np.array_equal(lab[off[i]:off[i + 1]], single), f"instance {i}"
+
+
+def test_autotune_remembers_shape(handle, devmem):
+    x, info = dclust.synth_config(3, n=200000)
+    p = dclust.make_params(3, info["eps"], info["minpts"], precision=dclust.FP32_GUARD)
+    d_x = devmem.upload(np.ascontiguousarray(x))
+    tuned = handle.autotune_device(d_x, [x.shape[0]], [p])[0]
+    q = handle.autotune(x.shape[0], dclust.make_params(3, info["eps"], info["minpts"],
+                                                       precision=dclust.FP32_GUARD))
+    assert (q.seed_list_size, q.blocks, q.block_threads) == \
+        (tuned.seed_list_size, tuned.blocks, tuned.block_threads)
+    _check(handle, x, info["eps"], info["minpts"], dclust.FP32_GUARD,
+           seeds=q.seed_list_size, blocks=q.blocks, threads=q.block_threads)
+
+
+def test_neighbourhood_statistics_switch(handle, monkeypatch):
+    x, info = dclust.synth_config(3, n=100000)
+    handle.run(x, info["eps"], info["minpts"], precision=dclust.FP32_GUARD)
+    assert handle.last_stats["cand_total"] == 0
+    monkeypatch.setenv("DCLUST_NBR_STATS", "1")
+    handle.run(x, info["eps"], info["minpts"], precision=dclust.FP32_GUARD)
+    st = handle.last_stats
+    assert st["cand_total"] >= st["n_points"] and st["nbr_cells_total"] >= st["n_cells"]
