@@ -56,6 +56,15 @@ struct dclust_handle {
   // launch shapes found by dclust_autotune_device, keyed by (dimension mix,
   // log2 of the total point count, precision form)
   std::map<std::tuple<int, int, int>, dclust_params> tuned;
+  // replay of repeated identical calls
+  std::vector<uint8_t> prev_sig;   // launch state of the last eager run
+  int prev_npass = 0;              // its expansion passes
+  long long prev_nlines = 0;       // its neighbour-line count
+  hipGraphExec_t gexec = nullptr;  // captured run
+  std::vector<uint8_t> gsig;       // launch state it was captured for
+  int gpasses = 0;                 // expansion passes it holds
+  bool graph_broken = false;       // capture failed on this device: eager only
+  bool graph_off = false;          // eager only (while autotuning)
 
   template <class T>
   T* get(int slot, size_t count, hipError_t* err) {
@@ -91,12 +100,14 @@ enum Slot {
   S_DTOT, S_PASS, S_CSTART, S_CKEY, S_CINST, S_CM, S_ROWS, S_CELLOF, S_X0, S_X1, S_X2,
   S_G0, S_G1, S_G2, S_ORIG, S_CORE, S_LABEL, S_PARENT, S_COMPMIN, S_PAIRS, S_Q0, S_Q1,
   S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT, S_PAIRCNT, S_SEGPOS, S_LINES, S_AMASK,
-  S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP, S_ADJPTS, S_CORECELLS, S_TUNELAB, S_STAT
+  S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP, S_ADJPTS, S_CORECELLS, S_TUNELAB, S_STAT, S_GLINES
 };
 
 // line table entries (8 bytes each) above which neighbour lines are found
 // by binary search instead
 constexpr long long kMaxLineTable = 1LL << 25;
+// expansion passes a captured run may hold; runs that need more stay eager
+constexpr int kGraphMaxPasses = 8;
 
 int check_params(const dclust_params* p) {
   if (!p) return DCLUST_E_PARAM;
@@ -162,6 +173,212 @@ RunShape pick_shape(dclust_handle* h, const dclust_params& p, int64_t n, bool f3
   (void)n;
   (void)f32;
   return r;
+}
+
+bool graph_enabled(const dclust_handle* h) {
+  if (h->graph_off || h->graph_broken) return false;
+  const char* e = std::getenv("DCLUST_GRAPH");
+  return !(e && e[0] == '0');
+}
+
+void drop_graph(dclust_handle* h) {
+  if (h->gexec) {
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipGraphExecDestroy(h->gexec);
+  }
+  h->gexec = nullptr;
+  h->gsig.clear();
+  h->gpasses = 0;
+}
+
+// Everything a launch sequence depends on besides the coordinates: the
+// device state (sizes, buffer and input/output pointers, switches), the
+// launch shape, the precision form and the instance and chunk tables.
+std::vector<uint8_t> launch_signature(const Dev& d, const RunShape& r, bool f32,
+                                      const std::vector<Inst>& inst,
+                                      const std::vector<Chunk>& chunks) {
+  std::vector<uint8_t> v;
+  auto put = [&v](const void* p, size_t nb) {
+    const uint8_t* b = (const uint8_t*)p;
+    v.insert(v.end(), b, b + nb);
+  };
+  put(&d, sizeof(d));
+  const int64_t sh[8] = {r.seeds, r.threads, r.blocks, r.waves, r.pair_per, r.budget,
+                         r.pair_cap, f32 ? 1 : 0};
+  put(sh, sizeof(sh));
+  put(inst.data(), sizeof(Inst) * inst.size());
+  put(chunks.data(), sizeof(Chunk) * chunks.size());
+  return v;
+}
+
+// Phase event; inside a stream capture it becomes a node of the graph.
+hipError_t mark(dclust_handle* h, int e, bool graph) {
+  return graph ? hipEventRecordWithFlags(h->ev[e], h->stream, hipEventRecordExternal)
+               : hipEventRecord(h->ev[e], h->stream);
+}
+
+// counters, bounding boxes, grid set-up
+void enqueue_head(const Dev& d, int nchunks, hipStream_t s) {
+  launch_fill_i64(d.ctl, C_NCTL, 0, s);
+  launch_fill_i64(d.stat, (int64_t)C_NCTL * kStripes, 0, s);
+  launch_fill_i32(d.pass_scatter, kMaxPasses, 0, s);
+  launch_bbox_init(d, s);
+  launch_index_head(d, nchunks, s);
+}
+
+// per-cell state, core flags, near adjacency, seeding segments
+hipError_t enqueue_core(dclust_handle* h, const Dev& d, const RunShape& shape, bool f32,
+                        bool graph) {
+  const hipStream_t s = h->stream;
+  const int64_t n = d.n;
+  hipError_t e;
+  if ((e = mark(h, E_INDEX, graph)) != hipSuccess) return e;
+  launch_fill_i32(d.cell_label, n, kUnclaimed, s);
+  launch_iota(d.parent, n, s);
+  launch_fill_i32(d.compmin, n, kNoKey, s);
+  launch_fill_u8(d.flag, n, 0, s);
+  launch_fill_i32(d.pair_cnt, shape.waves, 0, s);
+  launch_fill_i32(d.cell_ncore, n, 0, s);
+  // the core (neighbour-count) kernel is bracketed by its own events
+  if ((e = mark(h, E_CORE0, graph)) != hipSuccess) return e;
+  launch_core(d, f32, s);
+  if ((e = mark(h, E_CORE, graph)) != hipSuccess) return e;
+  launch_adj(d, f32, h->num_cu * 16, s);
+  if ((e = mark(h, E_ADJ, graph)) != hipSuccess) return e;
+  launch_seg_init(d, shape.waves, s);
+  return hipSuccess;
+}
+
+void enqueue_pass(const Dev& d, const RunShape& shape, int pass, hipStream_t s) {
+  launch_pass_prep(d, pass, s);
+  launch_expand(d, pass, shape.blocks, shape.seeds, shape.threads, shape.budget, s);
+  launch_pass_carry(d, pass, s);
+}
+
+// chain merge, far pairs, border points, canonical labels
+hipError_t enqueue_tail(dclust_handle* h, const Dev& d, bool f32, int grid_cells, bool graph) {
+  const hipStream_t s = h->stream;
+  hipError_t e;
+  if ((e = mark(h, E_EXPAND, graph)) != hipSuccess) return e;
+  launch_merge(d, grid_cells, s);
+  if ((e = mark(h, E_MERGE, graph)) != hipSuccess) return e;
+  launch_far(d, f32, grid_cells, s);
+  if ((e = mark(h, E_FAR, graph)) != hipSuccess) return e;
+  launch_border(d, f32, s);
+  if ((e = mark(h, E_BORDER, graph)) != hipSuccess) return e;
+  launch_relabel(d, d.n, s);
+  if ((e = mark(h, E_END, graph)) != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+// A whole run with no host read in between, captured into a graph: every
+// radix pass is launched (passes beyond the key width exit on the device)
+// and `passes` expansion passes (a pass with nothing pending exits at once).
+hipError_t capture_run(dclust_handle* h, const Dev& d, const RunShape& shape, bool f32,
+                       int nchunks, int grid_cells, int passes, hipGraphExec_t* out) {
+  const hipStream_t s = h->stream;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  hipError_t body = mark(h, E_START, true);
+  if (body == hipSuccess) {
+    enqueue_head(d, nchunks, s);
+    launch_index_tail(d, d.n, nchunks, grid_cells, kMaxPasses, s);
+    body = enqueue_core(h, d, shape, f32, true);
+  }
+  for (int p = 0; body == hipSuccess && p < passes; ++p) {
+    enqueue_pass(d, shape, p, s);
+    if (p == 0) body = mark(h, E_PASS0, true);
+  }
+  // work still pending after the last pass: the rest of the graph returns at
+  // once and the host runs the call eagerly
+  if (body == hipSuccess) launch_expand_check(d, passes, s);
+  if (body == hipSuccess) body = enqueue_tail(h, d, f32, grid_cells, true);
+  e = hipStreamEndCapture(s, &g);
+  if (body != hipSuccess) e = body;
+  if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  return e;
+}
+
+// Control counters and their striped parts, after the stream drains.
+hipError_t read_counters(const Dev& d, hipStream_t s, long long* ctl_h,
+                         std::vector<long long>& stat_h) {
+  hipError_t e = hipMemcpyAsync(ctl_h, d.ctl, sizeof(long long) * C_NCTL,
+                                hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(stat_h.data(), d.stat, sizeof(long long) * stat_h.size(),
+                       hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e;
+}
+
+// Per-instance error codes of the grid set-up; *first = the first non-zero.
+hipError_t read_inst_errors(const Dev& d, int32_t ninst, int32_t* inst_err, int* first) {
+  std::vector<Inst> got(ninst);
+  hipError_t e = hipMemcpy(got.data(), d.inst, sizeof(Inst) * ninst, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return e;
+  *first = 0;
+  for (int32_t i = 0; i < ninst; ++i) {
+    if (inst_err) inst_err[i] = got[i].err;
+    if (got[i].err && !*first) *first = got[i].err;
+  }
+  return hipSuccess;
+}
+
+// Records the run's device state for dclust_last_core_flags and fills the
+// statistics (counters summed over their stripes, phase times from events).
+void finish_run(dclust_handle* h, const Dev& d, const RunShape& shape, long long* ctl_h,
+                const std::vector<long long>& stat_h, int npass, bool replayed,
+                dclust_stats* stats) {
+  for (int c = 0; c < C_NCTL; ++c)
+    for (int k = 0; k < kStripes; ++k) ctl_h[c] += stat_h[(size_t)c * kStripes + k];
+  h->last_n = d.n;
+  h->last_core = d.core;
+  h->last_orig = d.orig;
+  h->last_core_out = d.core_out;
+  if (!stats) return;
+  stats->t_index_ms = ev_ms(h, E_START, E_CORE0);
+  stats->t_core_ms = ev_ms(h, E_CORE0, E_CORE);
+  stats->t_adj_ms = ev_ms(h, E_CORE, E_ADJ);
+  stats->t_expand_ms = ev_ms(h, E_ADJ, E_EXPAND);
+  stats->t_merge_ms = ev_ms(h, E_EXPAND, E_MERGE);
+  stats->t_far_ms = ev_ms(h, E_MERGE, E_FAR);
+  stats->t_border_ms = ev_ms(h, E_FAR, E_BORDER);
+  stats->t_relabel_ms = ev_ms(h, E_BORDER, E_END);
+  stats->t_total_ms = ev_ms(h, E_START, E_END);
+  stats->key_bits = (int32_t)ctl_h[C_KEYBITS];
+  stats->n_cells = ctl_h[C_NCELL];
+  stats->n_core = ctl_h[C_NCORE];
+  stats->n_noise = ctl_h[C_NNOISE];
+  stats->n_clusters = ctl_h[C_NCLUST];
+  stats->n_chains = ctl_h[C_NCHAIN];
+  stats->n_collisions = ctl_h[C_NPAIR];
+  stats->n_overflow = ctl_h[C_OVF];
+  stats->n_passes = npass;
+  stats->cand_total = ctl_h[C_CAND];
+  stats->nbr_cells_total = ctl_h[C_NBRC];
+  stats->band_rechecks = ctl_h[C_BAND];
+  stats->band_flips = ctl_h[C_FLIP];
+  stats->asym_pairs = ctl_h[C_ASYM];
+  stats->pairs_evaluated = ctl_h[C_EVAL];
+  stats->band_rechecks_core = ctl_h[C_BAND_CORE];
+  stats->rec_overflow = ctl_h[C_RECOVF];
+  stats->diag[0] = ctl_h[C_POPS];
+  stats->diag[1] = ctl_h[C_DENSE];
+  stats->diag[2] = ctl_h[C_NHEAVY];
+  stats->diag[3] = ctl_h[C_EDGES];
+  stats->diag[4] = (int64_t)(1000.0 * ev_ms(h, E_ADJ, E_PASS0));
+  stats->diag[5] = ctl_h[C_BUSY_SUM];
+  stats->diag[6] = ctl_h[C_BUSY_MAX];
+  stats->diag[7] = shape.waves;
+  stats->diag[8] = ctl_h[C_ADJ_TESTS];
+  stats->diag[9] = ctl_h[C_ADJ_FREE];
+  stats->diag[10] = ctl_h[C_FAR_TESTS];
+  stats->diag[11] = ctl_h[C_FAR_JOINS];
+  stats->diag[12] = ctl_h[C_POPS_MAX];
+  stats->diag[13] = ctl_h[C_SEEDS_MAX];
+  stats->diag[14] = replayed ? 1 : 0;
 }
 
 #define HIPCHK(x)                          \
@@ -306,141 +523,115 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
   }
   d.labels_out = d_labels;
 
+  // A call whose launch state equals the previous call's (sizes, parameters,
+  // pointers) is captured once into a graph and replayed from then on.
+  std::vector<uint8_t> sig;
+  const bool graph_on = graph_enabled(h);
+  if (graph_on) sig = launch_signature(d, shape, f32, inst, chunks);
+  if (h->gexec && (!graph_on || sig != h->gsig)) drop_graph(h);
+  if (graph_on && !h->gexec && !h->prev_sig.empty() && sig == h->prev_sig &&
+      h->prev_npass < kGraphMaxPasses) {
+    Dev dg = d;
+    // line table sized from the previous run; a larger line count falls back
+    // to the binary search on the device
+    const long long cap = std::min<long long>(
+        kMaxLineTable, std::max<long long>(h->prev_nlines + h->prev_nlines / 4, 1024));
+    hipError_t ae = hipSuccess;
+    dg.lines = h->get<int2>(S_GLINES, (size_t)cap, &ae);
+    if (dg.lines) {
+      dg.line_cap = cap;
+      const int gp = std::max(2, h->prev_npass + 1);
+      hipGraphExec_t ge = nullptr;
+      if (capture_run(h, dg, shape, f32, nchunks, grid_cells, gp, &ge) == hipSuccess) {
+        h->gexec = ge;
+        h->gsig = sig;
+        h->gpasses = gp;
+      } else {
+        (void)hipGetLastError();
+        h->graph_broken = true;  // capture unsupported here: stay eager
+      }
+    }
+  }
+
   {
     long long ctl_h[C_NCTL];
-    float t_pass0_ms = 0.f;
-    int passes = 0;
+    std::vector<long long> stat_h((size_t)C_NCTL * kStripes);
+    if (h->gexec) {
+      HIPCHK(hipMemcpyAsync(d.inst, inst.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync((void*)d.chunks, chunks.data(), sizeof(Chunk) * nchunks,
+                            hipMemcpyHostToDevice, s));
+      HIPCHK(hipGraphLaunch(h->gexec, s));
+      HIPCHK(read_counters(d, s, ctl_h, stat_h));
+      if (!ctl_h[C_ABANDON]) {
+        if (ctl_h[C_ERR]) {
+          int first = 0;
+          HIPCHK(read_inst_errors(d, ninst, inst_err, &first));
+          if (ninst == 1 || !inst_err) return first ? first : DCLUST_E_PARAM;
+          rc = first;
+        }
+        finish_run(h, d, shape, ctl_h, stat_h, (int)std::max<long long>(1, ctl_h[C_LASTPASS]),
+                   true, stats);
+        return rc;
+      }
+      // the expansion needed more passes than the graph holds: run eagerly
+      // (the next identical call captures a graph with more passes)
+      drop_graph(h);
+    }
+
     int npass = 0;
     HIPCHK(hipMemcpyAsync(d.inst, inst.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync((void*)d.chunks, chunks.data(), sizeof(Chunk) * nchunks,
                           hipMemcpyHostToDevice, s));
-    HIPCHK(hipEventRecord(h->ev[E_START], s));
-    launch_fill_i64(d.ctl, C_NCTL, 0, s);
-    launch_fill_i64(d.stat, (int64_t)C_NCTL * kStripes, 0, s);
-    launch_fill_i32(d.pass_scatter, kMaxPasses, 0, s);
-    launch_bbox_init(d, s);
-    launch_index_head(d, nchunks, s);
+    HIPCHK(mark(h, E_START, false));
+    enqueue_head(d, nchunks, s);
     HIPCHK(hipMemcpyAsync(ctl_h, d.ctl, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (ctl_h[C_ERR]) {
-      HIPCHK(hipMemcpy(inst.data(), d.inst, sizeof(Inst) * ninst, hipMemcpyDeviceToHost));
       int first = 0;
-      for (int32_t i = 0; i < ninst; ++i) {
-        if (inst_err) inst_err[i] = inst[i].err;
-        if (inst[i].err && !first) first = inst[i].err;
-      }
+      HIPCHK(read_inst_errors(d, ninst, inst_err, &first));
       if (ninst == 1 || !inst_err) return first ? first : DCLUST_E_PARAM;
       rc = first;  // batched with per-instance errors: keep going
     }
-    passes = (int)((ctl_h[C_KEYBITS] + 7) / 8);
+    const int passes = (int)((ctl_h[C_KEYBITS] + 7) / 8);
+    const long long nlines = ctl_h[C_NLINES];
     if (stats) stats->key_bits = (int32_t)ctl_h[C_KEYBITS];
     // dense line table when it is small enough, else binary search
     d.line_cap = 0;
-    if (ctl_h[C_NLINES] <= kMaxLineTable) {
-      d.lines = h->get<int2>(S_LINES, (size_t)std::max<long long>(ctl_h[C_NLINES], 1), &herr);
+    if (nlines <= kMaxLineTable) {
+      d.lines = h->get<int2>(S_LINES, (size_t)std::max<long long>(nlines, 1), &herr);
       if (!d.lines) return DCLUST_E_OOM;
-      d.line_cap = ctl_h[C_NLINES];
+      d.line_cap = nlines;
     }
     launch_index_tail(d, ntot, nchunks, grid_cells, passes, s);
-    HIPCHK(hipEventRecord(h->ev[E_INDEX], s));
-
-    launch_fill_i32(d.cell_label, ntot, kUnclaimed, s);
-    launch_iota(d.parent, ntot, s);
-    launch_fill_i32(d.compmin, ntot, kNoKey, s);
-    launch_fill_u8(d.flag, ntot, 0, s);
-    launch_fill_i32(d.pair_cnt, shape.waves, 0, s);
-    launch_fill_i32(d.cell_ncore, ntot, 0, s);
-    // the core (neighbour-count) kernel is bracketed by its own events
-    HIPCHK(hipEventRecord(h->ev[E_CORE0], s));
-    launch_core(d, f32, s);
-    HIPCHK(hipEventRecord(h->ev[E_CORE], s));
-    launch_adj(d, f32, h->num_cu * 16, s);
-    HIPCHK(hipEventRecord(h->ev[E_ADJ], s));
+    HIPCHK(enqueue_core(h, d, shape, f32, false));
 
     // expansion passes until no seed is pending: pass p drains the queue of
     // pass p-1 (overflowed seed lists, lists left when a wave's pop budget
     // ran out) and continues every unfinished seeding segment
-    launch_seg_init(d, shape.waves, s);
     for (npass = 0; npass < kMaxExpandPasses; ++npass) {
-      launch_pass_prep(d, npass, s);
-      launch_expand(d, npass, shape.blocks, shape.seeds, shape.threads, shape.budget, s);
-      launch_pass_carry(d, npass, s);
-      if (npass == 0) HIPCHK(hipEventRecord(h->ev[E_PASS0], s));
+      enqueue_pass(d, shape, npass, s);
+      if (npass == 0) HIPCHK(mark(h, E_PASS0, false));
       HIPCHK(hipMemcpyAsync(ctl_h, d.ctl, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
       if (ctl_h[C_Q0_N + ((npass + 1) & 1)] == 0 && ctl_h[C_SEGS_LEFT] == 0) break;
     }
     ++npass;
-    HIPCHK(hipEventRecord(h->ev[E_EXPAND], s));
-    launch_merge(d, grid_cells, s);
-    HIPCHK(hipEventRecord(h->ev[E_MERGE], s));
-    launch_far(d, f32, grid_cells, s);
-    HIPCHK(hipEventRecord(h->ev[E_FAR], s));
-    launch_border(d, f32, s);
-    HIPCHK(hipEventRecord(h->ev[E_BORDER], s));
-    launch_relabel(d, ntot, s);
-    HIPCHK(hipEventRecord(h->ev[E_END], s));
-    HIPCHK(hipGetLastError());
-    std::vector<long long> stat_h((size_t)C_NCTL * kStripes);
-    HIPCHK(hipMemcpyAsync(ctl_h, d.ctl, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(stat_h.data(), d.stat, sizeof(long long) * stat_h.size(),
-                          hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    for (int c = 0; c < C_NCTL; ++c)
-      for (int k = 0; k < kStripes; ++k) ctl_h[c] += stat_h[(size_t)c * kStripes + k];
-    t_pass0_ms = ev_ms(h, E_ADJ, E_PASS0);
-    h->last_n = ntot;
-    h->last_core = d.core;
-    h->last_orig = d.orig;
-    h->last_core_out = d.core_out;
-    if (stats) {
-      stats->t_index_ms = ev_ms(h, E_START, E_CORE0);
-      stats->t_core_ms = ev_ms(h, E_CORE0, E_CORE);
-      stats->t_adj_ms = ev_ms(h, E_CORE, E_ADJ);
-      stats->t_expand_ms = ev_ms(h, E_ADJ, E_EXPAND);
-      stats->t_merge_ms = ev_ms(h, E_EXPAND, E_MERGE);
-      stats->t_far_ms = ev_ms(h, E_MERGE, E_FAR);
-      stats->t_border_ms = ev_ms(h, E_FAR, E_BORDER);
-      stats->t_relabel_ms = ev_ms(h, E_BORDER, E_END);
-      stats->t_total_ms = ev_ms(h, E_START, E_END);
-      stats->n_cells = ctl_h[C_NCELL];
-      stats->n_core = ctl_h[C_NCORE];
-      stats->n_noise = ctl_h[C_NNOISE];
-      stats->n_clusters = ctl_h[C_NCLUST];
-      stats->n_chains = ctl_h[C_NCHAIN];
-      stats->n_collisions = ctl_h[C_NPAIR];
-      stats->n_overflow = ctl_h[C_OVF];
-      stats->n_passes = npass;
-      stats->cand_total = ctl_h[C_CAND];
-      stats->nbr_cells_total = ctl_h[C_NBRC];
-      stats->band_rechecks = ctl_h[C_BAND];
-      stats->band_flips = ctl_h[C_FLIP];
-      stats->asym_pairs = ctl_h[C_ASYM];
-      stats->pairs_evaluated = ctl_h[C_EVAL];
-      stats->band_rechecks_core = ctl_h[C_BAND_CORE];
-      stats->rec_overflow = ctl_h[C_RECOVF];
-      stats->diag[0] = ctl_h[C_POPS];
-      stats->diag[1] = ctl_h[C_DENSE];
-      stats->diag[2] = ctl_h[C_NHEAVY];
-      stats->diag[3] = ctl_h[C_EDGES];
-      stats->diag[4] = (int64_t)(1000.0 * t_pass0_ms);
-      stats->diag[5] = ctl_h[C_BUSY_SUM];
-      stats->diag[6] = ctl_h[C_BUSY_MAX];
-      stats->diag[7] = shape.waves;
-      stats->diag[8] = ctl_h[C_ADJ_TESTS];
-      stats->diag[9] = ctl_h[C_ADJ_FREE];
-      stats->diag[10] = ctl_h[C_FAR_TESTS];
-      stats->diag[11] = ctl_h[C_FAR_JOINS];
-      stats->diag[12] = ctl_h[C_POPS_MAX];
-      stats->diag[13] = ctl_h[C_SEEDS_MAX];
-    }
+    HIPCHK(enqueue_tail(h, d, f32, grid_cells, false));
+    HIPCHK(read_counters(d, s, ctl_h, stat_h));
+    finish_run(h, d, shape, ctl_h, stat_h, npass, false, stats);
     if (npass >= kMaxExpandPasses) return DCLUST_E_CAPACITY;
+    if (graph_on) {
+      h->prev_sig.swap(sig);
+      h->prev_npass = npass;
+      h->prev_nlines = nlines;
+    }
     return rc;
   }
 fail:
   (void)hipGetLastError();
   if (stats) stats->hip_error = (int32_t)herr;
   h->last_err = (int)herr;
+  h->prev_sig.clear();
   return DCLUST_E_HIP;
 }
 
@@ -502,6 +693,7 @@ void dclust_destroy(dclust_handle* h) {
   if (!h) return;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  drop_graph(h);
   for (auto& b : h->bufs)
     if (b.p) (void)hipFree(b.p);
   for (auto& e : h->ev)
@@ -644,6 +836,12 @@ int dclust_autotune_device(dclust_handle* h, int32_t count, const double* d_xyz,
   const int threads_c[] = {128, 256};
   const int per_cu_c[] = {4, 8, 16};
   std::vector<dclust_params> trial(params, params + count);
+  // candidates are timed by eager runs (no capture per candidate shape)
+  struct Eager {
+    dclust_handle* h;
+    ~Eager() { h->graph_off = false; }
+  } eager{h};
+  h->graph_off = true;
   double best = 1e30;
   dclust_params best_p = params[0];
   for (int sd : seeds_c)

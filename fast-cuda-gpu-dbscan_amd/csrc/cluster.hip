@@ -846,6 +846,7 @@ __global__ __launch_bounds__(64) void k_adj_heavy(Dev d) {
 // Component of every core cell after the expansion and merge (the root of
 // its chain), and the compacted list of core cells.
 __global__ __launch_bounds__(1024) void k_cell_comp(Dev d) {
+  if (abandoned(d)) return;
   const int64_t C = d.ctl[C_NCELL];
   {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -931,6 +932,7 @@ __device__ long long far_cell(const Dev& d, const Inst& I, const Par& P, int32_t
 
 template <bool F32>
 __global__ __launch_bounds__(256) void k_far(Dev d) {
+  if (abandoned(d)) return;
   __shared__ long long s_pre[kStripes + 1];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = i < slist_prefix(d.core_cells, s_pre);
@@ -955,6 +957,7 @@ __global__ __launch_bounds__(256) void k_far(Dev d) {
 // Roots of every chain after the far-pair unions, and the minimum core
 // input index of each component (the minimum over its chains).
 __global__ __launch_bounds__(256) void k_flatten_all(Dev d) {
+  if (abandoned(d)) return;
   const long long nc = d.ctl[C_NCHAIN];
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc;
        c += (long long)gridDim.x * blockDim.x) {
@@ -982,6 +985,7 @@ constexpr int kSeen = 256;   // per-wave table of recently recorded collision pa
 __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budget) {
   extern __shared__ int32_t s_lists[];
   __shared__ unsigned long long s_seen[4][kSeen];
+  if (d.ctl[C_IDLE]) return;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   const int nwb = blockDim.x >> 6;
   const int64_t gw = (int64_t)xcd_block(blockIdx.x, gridDim.x) * nwb + wv;
@@ -1169,14 +1173,24 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
     atomicAdd((unsigned long long*)&d.ctl[C_POPS], (unsigned long long)pops);
     atomicMax((unsigned long long*)&d.ctl[C_POPS_MAX], (unsigned long long)pops);
     atomicMax((unsigned long long*)&d.ctl[C_SEEDS_MAX], (unsigned long long)seeds);
+    if (pops) atomicMax((unsigned long long*)&d.ctl[C_LASTPASS], (unsigned long long)(pass + 1));
   }
   stat_add(d, C_OVF, ovf);
   stat_add(d, C_EDGES, edges);
 }
 
+// After the last of a fixed number of expansion passes: work still pending
+// (queue or seeding segments) marks the run as abandoned, and the kernels
+// after the expansion return at once (the host runs the call again).
+__global__ void k_expand_check(Dev d, int passes) {
+  d.ctl[C_ABANDON] = d.ctl[C_Q0_N + (passes & 1)] != 0 || d.ctl[C_SEGS_LEFT] != 0;
+}
+
 // Resets the counters of the queue that pass `pass` fills and of the head
-// of the queue it drains.
+// of the queue it drains, and marks the pass idle when nothing is pending.
 __global__ void k_pass_prep(Dev d, int pass) {
+  // nothing left by the previous pass: the waves of this one return at once
+  d.ctl[C_IDLE] = pass > 0 && d.ctl[C_Q0_N + (pass & 1)] == 0 && d.ctl[C_SEGS_LEFT] == 0;
   d.ctl[C_Q0_N + ((pass + 1) & 1)] = 0;
   d.ctl[C_Q0_HEAD + (pass & 1)] = 0;
   d.ctl[C_SEGS_LEFT] = 0;
@@ -1229,6 +1243,7 @@ __device__ __forceinline__ int32_t lds_find(int32_t* par, int32_t x) {
 __device__ __forceinline__ int64_t pair_waves(const Dev& d) { return d.pair_cap / d.pair_per; }
 
 __global__ __launch_bounds__(1024) void k_merge_lds(Dev d) {
+  if (abandoned(d)) return;
   __shared__ int32_t par[kLdsChains];
   __shared__ int32_t s_pre[kMergeChunk + 1];
   __shared__ int32_t s_wsum[16];
@@ -1297,6 +1312,7 @@ __global__ __launch_bounds__(1024) void k_merge_lds(Dev d) {
 }
 
 __global__ __launch_bounds__(256) void k_union_pairs(Dev d) {
+  if (abandoned(d)) return;
   if (d.ctl[C_NCHAIN] <= kLdsChains) return;
   const int64_t nw = pair_waves(d);
   const int64_t wid = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
@@ -1311,6 +1327,7 @@ __global__ __launch_bounds__(256) void k_union_pairs(Dev d) {
 }
 
 __global__ __launch_bounds__(256) void k_flatten(Dev d) {
+  if (abandoned(d)) return;
   const long long nc = d.ctl[C_NCHAIN];
   if (nc <= kLdsChains) return;
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc;
@@ -1324,6 +1341,7 @@ __device__ __forceinline__ int32_t core_chain(const Dev& d, int64_t i) {
 }
 
 __global__ __launch_bounds__(256) void k_keys(Dev d) {
+  if (abandoned(d)) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < d.n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int32_t li = core_chain(d, i);
@@ -1411,6 +1429,7 @@ __device__ int32_t border_lane(const Dev& d, const Par& P, int32_t p, int32_t c,
 
 template <bool F32>
 __global__ __launch_bounds__(256) void k_border(Dev d) {
+  if (abandoned(d)) return;
   const int64_t pl = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   bool act = pl < d.n;
   const int32_t p = act ? (int32_t)pl : 0;
@@ -1446,6 +1465,7 @@ constexpr int RK_I = 16;
 constexpr int RK_TILE = RK_T * RK_I;
 
 __global__ __launch_bounds__(RK_T) void k_rank_count(Dev d) {
+  if (abandoned(d)) return;
   const int64_t t0 = (int64_t)blockIdx.x * RK_TILE;
   int c = 0;
   for (int r = 0; r < RK_I; ++r) {
@@ -1459,6 +1479,7 @@ __global__ __launch_bounds__(RK_T) void k_rank_count(Dev d) {
 }
 
 __global__ __launch_bounds__(RK_T) void k_rank_write(Dev d) {
+  if (abandoned(d)) return;
   const int64_t t0 = (int64_t)blockIdx.x * RK_TILE;
   __shared__ int s_w[4];
   __shared__ int64_t s_run;
@@ -1485,6 +1506,7 @@ __global__ __launch_bounds__(RK_T) void k_rank_write(Dev d) {
 }
 
 __global__ __launch_bounds__(256) void k_output(Dev d) {
+  if (abandoned(d)) return;
   long long noise = 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); i0 < d.n; i0 += stride) {
@@ -1535,6 +1557,10 @@ void launch_pass_prep(const Dev& d, int pass, hipStream_t s) {
 
 void launch_pass_carry(const Dev& d, int pass, hipStream_t s) {
   hipLaunchKernelGGL(k_pass_carry, dim3(1), dim3(256), 0, s, d, pass);
+}
+
+void launch_expand_check(const Dev& d, int passes, hipStream_t s) {
+  hipLaunchKernelGGL(k_expand_check, dim3(1), dim3(1), 0, s, d, passes);
 }
 
 void launch_seg_init(const Dev& d, int64_t waves, hipStream_t s) {

@@ -186,7 +186,14 @@ __global__ void k_grid_setup(Dev d) {
         emin = fmin(emin, mx[k] - mn[k]);
       }
       I.maxabs = empty ? 0.0 : unord_d(bb[6]);
-      if (bb[7]) I.err = -1;  // DCLUST_E_PARAM: non-finite input
+      if (bb[7]) {
+        // DCLUST_E_PARAM: non-finite input.  The instance keeps a one-cell
+        // grid (every point in cell 0) so that every kernel stays in bounds;
+        // its result is discarded.
+        I.err = -1;
+        for (int k = 0; k < dim; ++k) mn[k] = mx[k] = 0.0;
+        I.maxabs = 0.0;
+      }
       if (I.r > 0) {
         double wr = emin / (double)I.r;
         if (!(wr >= I.eps) && I.err == 0 && !empty) I.err = -2;  // DCLUST_E_BINWIDTH

@@ -132,10 +132,15 @@ enum Ctl : int {
   C_ADJ_FREE,   // adjacency bits set without a test (cell inside the eps ball)
   C_FAR_TESTS,  // pair tests of the far-pair pass
   C_FAR_JOINS,  // far pairs that joined two components
+  C_LASTPASS,   // 1 + the last expansion pass that popped a cell
   C_POPS_MAX,   // most cells popped by one expansion wave in one pass
   C_SEEDS_MAX,  // most chains started by one expansion wave in one pass
-  C_NCTL = 40
+  C_ABANDON,    // expansion left work pending in a run with a fixed pass
+                // count: the kernels after it return at once
+  C_IDLE,       // nothing pending for the current expansion pass
+  C_NCTL = 48
 };
+static_assert(C_IDLE < C_NCTL, "control block too small");
 
 // Statistics counters are striped: a wave adds to one of kStripes copies.
 constexpr int kStripes = 64;
@@ -303,6 +308,9 @@ __device__ __forceinline__ void wave_add(long long* p, long long v) {
 // of the counter, picked by the wave's position in the launch, so that a
 // launch of many waves does not serialise on one address.  The host sums
 // the copies.
+// set when a run with a fixed number of expansion passes left work pending
+__device__ __forceinline__ bool abandoned(const Dev& d) { return d.ctl[C_ABANDON] != 0; }
+
 __device__ __forceinline__ void stat_add(const Dev& d, int ctr, long long v) {
   const long long s = wave_sum(v);
   if (lane_id() == 0 && s != 0) {

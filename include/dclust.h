@@ -116,8 +116,8 @@ typedef struct dclust_stats {
    * near-adjacency kernel, [9] adjacency bits set without a test, [10]
    * pair tests of the far-pair pass, [11] far pairs that joined two
    * components, [12] most cells popped by one expansion wave in a pass,
-   * [13] most chains started by one expansion wave in a pass, [14]-[15]
-   * reserved */
+   * [13] most chains started by one expansion wave in a pass, [14] 1 if
+   * the run was a replay of a captured graph, [15] reserved */
   int64_t diag[16];
 } dclust_stats;
 
@@ -142,7 +142,18 @@ int dclust_run(dclust_handle* h, const double* xyz, int64_t n,
                const dclust_params* p, int32_t* labels_out,
                dclust_stats* stats);
 
-/* Same with device-resident input and output (HIP device pointers). */
+/* Same with device-resident input and output (HIP device pointers).
+ *
+ * Repeated calls: a call whose sizes, parameters, pointers and environment
+ * switches all equal the previous call's is captured once into a hipGraph
+ * and replayed from then on (the coordinates may change between calls).
+ * A replay launches the whole run at once with no host read in between:
+ * every radix pass (passes beyond the key width exit on the device) and a
+ * fixed number of expansion passes, one more than the previous run needed.
+ * When the replay finds that the expansion needed more, the call reruns
+ * eagerly and the next identical call captures a longer graph.  The
+ * environment variable DCLUST_GRAPH=0 turns replay off.  This applies to
+ * every entry point that runs a clustering (dclust_run, the batched ones). */
 int dclust_run_device(dclust_handle* h, const double* d_xyz, int64_t n,
                       const dclust_params* p, int32_t* d_labels_out,
                       dclust_stats* stats);

@@ -1,6 +1,7 @@
 import os
 import sys
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -58,6 +59,10 @@ class DeviceMem:
         assert self.L.hipMalloc(C.byref(p), max(nbytes, 1)) == 0
         self.live.append(p.value)
         return p.value
+
+    def h2d(self, ptr, arr):
+        arr = np.ascontiguousarray(arr)
+        assert self.L.hipMemcpy(self.C.c_void_p(ptr), arr.ctypes.data, arr.nbytes, 1) == 0
 
     def download(self, arr, ptr):
         assert self.L.hipMemcpy(arr.ctypes.data, self.C.c_void_p(ptr), arr.nbytes, 2) == 0

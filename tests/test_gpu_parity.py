@@ -73,10 +73,11 @@ def test_random_blobs(handle, prec, dim, seed):
 @pytest.mark.parametrize("seeds", [1, 2, 7])
 def test_tiny_seed_list_forces_correction(handle, prec, seeds):
     # seed lists of 1-2 cells overflow constantly: claimed cells go to the
-    # correction queue and are expanded by later passes
+    # correction queue and are expanded by later passes (few expansion waves,
+    # so that each grows its chains instead of seeding one chain per cell)
     rng = np.random.default_rng(7)
     x = rng.normal(0, 1, (8000, 2))
-    st = _check(handle, x, 0.08, 4, prec, seeds=seeds)
+    st = _check(handle, x, 0.08, 4, prec, seeds=seeds, blocks=8)
     if seeds <= 2:
         assert st["n_overflow"] > 0 and st["n_passes"] > 1
 
@@ -214,3 +215,124 @@ def test_neighbourhood_statistics_switch(handle, monkeypatch):
     handle.run(x, info["eps"], info["minpts"], precision=dclust.FP32_GUARD)
     st = handle.last_stats
     assert st["cand_total"] >= st["n_points"] and st["nbr_cells_total"] >= st["n_cells"]
+
+
+def _blob_sample(k, n, dim=3):
+    r = np.random.default_rng(300 + k)
+    centres = r.uniform(0, 1, (10, dim))
+    x = centres[r.integers(0, 10, n)] + r.normal(0, 0.02, (n, dim))
+    x[: n // 10] = r.uniform(0, 1, (n // 10, dim))
+    return x
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_graph_replay(handle, devmem, prec, monkeypatch):
+    # repeated identical calls: the first runs eagerly, the second is captured
+    # into a graph and replayed, later ones replay it; the coordinates change
+    # between calls at the same device pointers, and a call of another size
+    # in between drops the graph
+    monkeypatch.delenv("DCLUST_GRAPH", raising=False)
+    n, eps, mp = 30000, 0.02, 5
+    p = dclust.make_params(3, eps, mp, precision=prec)
+    d_x = devmem.alloc(n * 3 * 8)
+    d_lab = devmem.alloc(n * 4)
+    seq = ["other", 0, 0, 1, 2, "other", 2, 3, 0]
+    replayed = []
+    for step, k in enumerate(seq):
+        if k == "other":
+            _check(handle, _blob_sample(9, n // 2), eps, mp, prec)
+            replayed.append(handle.last_stats["diag"][14])
+            continue
+        x = _blob_sample(k, n)
+        devmem.h2d(d_x, x)
+        st = handle.run_device(d_x, n, p, d_lab)
+        lab = devmem.download(np.empty(n, dtype=np.int32), d_lab)
+        ref, core, asym = oracle.fast_dbscan(x, eps, mp)
+        assert np.array_equal(lab, ref), f"call {step}: {int(np.sum(lab != ref))} labels differ"
+        assert np.array_equal(handle.core_flags(n), core), f"call {step}: core flags"
+        assert st["n_clusters"] == int(ref.max())
+        if st["diag"][14]:
+            # phase events are nodes of the graph
+            assert st["t_core_ms"] > 0 and st["t_total_ms"] >= st["t_core_ms"]
+        replayed.append(st["diag"][14])
+    # (the first call's outcome depends on the call before this test)
+    assert replayed[1:] == [0, 1, 1, 1, 0, 0, 1, 1]
+
+
+def test_graph_replay_switch(handle, monkeypatch):
+    monkeypatch.setenv("DCLUST_GRAPH", "0")
+    x = _blob_sample(4, 12000, dim=2)
+    for _ in range(3):
+        _check(handle, x, 0.01, 5, dclust.FP32_GUARD)
+        assert handle.last_stats["diag"][14] == 0
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_graph_replay_reruns_when_passes_run_out(handle, prec, monkeypatch):
+    # a graph holds one expansion pass more than the run before its capture
+    # needed; data that needs more abandons the replay on the device and the
+    # call runs eagerly; the next identical call captures a longer graph
+    monkeypatch.delenv("DCLUST_GRAPH", raising=False)
+    # every wave hands its list on after one seed batch: clustered data needs
+    # many passes
+    monkeypatch.setenv("DCLUST_EXPAND_BUDGET", "1")
+    dense, info = dclust.synth_config(3, n=120000)
+    eps, mp = info["eps"], info["minpts"]
+    rng = np.random.default_rng(12)
+    sparse = rng.uniform(0, 1000 * eps, dense.shape)  # isolated points: one pass
+    _check(handle, sparse[: len(sparse) // 2], eps, mp, prec, blocks=16)
+    out = []
+    for x in [sparse, sparse, dense, dense, dense]:
+        st = _check(handle, x, eps, mp, prec, blocks=16)
+        out.append((st["diag"][14], st["n_passes"]))
+    assert out[0] == (0, 1) and out[1] == (1, 1)
+    assert out[2][0] == 0 and out[2][1] > 2
+    longer = 1 if out[2][1] < 8 else 0
+    assert out[3][0] == longer and out[4][0] == longer
+
+
+def test_batched_instance_errors(handle):
+    # instances with non-finite coordinates, or with a bin width below eps,
+    # report their own error code; the others are clustered as if alone
+    xs, params = _batch(limit=40, max_n=20000)
+    assert len(xs) >= 5
+    nan = xs[1].copy()
+    nan[5, 0] = np.nan
+    inf = xs[2].copy()
+    inf[7, -1] = np.inf
+    narrow = dclust.make_params(xs[4].shape[1], params[4].eps, params[4].minpts, r=10**6,
+                                precision=dclust.FP32_GUARD)
+    labs, errs = handle.run_batched([xs[0], nan, inf, xs[3], xs[4]],
+                                    [params[0], params[1], params[2], params[3], narrow])
+    assert errs.tolist() == [0, dclust.E_PARAM, dclust.E_PARAM, 0, dclust.E_BINWIDTH]
+    for i in (0, 3):
+        ref = oracle.fast_dbscan(xs[i], params[i].eps, params[i].minpts)[0]
+        assert np.array_equal(labs[i], ref), f"instance {i}"
+
+
+def test_graph_replay_non_finite_input(handle, devmem, monkeypatch):
+    # a replay whose coordinates turn non-finite reports the error; the next
+    # replay with finite coordinates is exact again
+    monkeypatch.delenv("DCLUST_GRAPH", raising=False)
+    n, eps, mp = 20000, 0.02, 5
+    x = _blob_sample(5, n)
+    p = dclust.make_params(3, eps, mp, precision=dclust.FP32_GUARD)
+    d_x = devmem.alloc(n * 3 * 8)
+    d_lab = devmem.alloc(n * 4)
+    ref = oracle.fast_dbscan(x, eps, mp)[0]
+    _check(handle, x[: n // 2], eps, mp, dclust.FP32_GUARD)
+    bad = x.copy()
+    bad[123, 2] = np.nan
+    seen = []
+    for step, data in enumerate([x, x, bad, x]):
+        devmem.h2d(d_x, data)
+        if data is bad:
+            with pytest.raises(dclust.DclustError) as e:
+                handle.run_device(d_x, n, p, d_lab)
+            assert e.value.code == dclust.E_PARAM
+            continue
+        st = handle.run_device(d_x, n, p, d_lab)
+        lab = devmem.download(np.empty(n, dtype=np.int32), d_lab)
+        assert np.array_equal(lab, ref), f"call {step}"
+        seen.append(st["diag"][14])
+    assert seen == [0, 1, 1]
