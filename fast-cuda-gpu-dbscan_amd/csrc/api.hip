@@ -467,7 +467,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     hipError_t& E = herr;
     d.inst = h->get<Inst>(S_INST, ninst, &E);
     d.chunks = h->get<Chunk>(S_CHUNK, nchunks, &E);
-    d.bbox = h->get<unsigned long long>(S_BBOX, (size_t)ninst * 8, &E);
+    d.bbox = h->get<unsigned long long>(S_BBOX, (size_t)ninst * kBoxStripes * 8, &E);
     d.ctl = h->get<long long>(S_CTL, C_NCTL, &E);
     d.stat = h->get<long long>(S_STAT, (size_t)C_NCTL * kStripes, &E);
     d.key[0] = h->get<uint64_t>(S_KEY0, ntot, &E);

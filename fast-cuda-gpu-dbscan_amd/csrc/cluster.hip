@@ -905,15 +905,24 @@ __device__ long long far_cell(const Dev& d, const Inst& I, const Par& P, int32_t
     const int o1 = D == 3 ? kRowOff3[r][1] : 0;
     const bool far_l = o0 == 2 || o0 == -2 || o1 == 2 || o1 == -2;
     const int64_t line_delta = D == 3 ? ((int64_t)o0 * I.nk[1] + o1) * nkl : (int64_t)o0 * nkl;
+    // cells above A in another component (cell_comp is -1 for a cell
+    // without core points); in a near line only the cells two steps away
+    // along the line are far
     unsigned todo = 0u;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       const int32_t B = st + k;
       const bool up = k < len && B > A;
-      const int32_t nco = up ? d.cell_ncore[B] : 0;
       const int32_t comp = up ? d.cell_comp[B] : compA;
-      const int ol = (up && !far_l) ? (int)((int64_t)d.cell_key[B] - keyA - line_delta) : 2;
-      if (up && (ol == 2 || ol == -2) && nco > 0 && comp != compA) todo |= 1u << k;
+      if (comp >= 0 && comp != compA) todo |= 1u << k;
+    }
+    if (todo && !far_l) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+        if (todo & (1u << k)) {
+          const int ol = (int)((int64_t)d.cell_key[st + k] - keyA - line_delta);
+          if (ol != 2 && ol != -2) todo &= ~(1u << k);
+        }
     }
     while (todo) {
       const int k = __ffs(todo) - 1;
@@ -1282,24 +1291,35 @@ __global__ __launch_bounds__(1024) void k_merge_lds(Dev d) {
     if (threadIdx.x == blockDim.x - 1) s_pre[kMergeChunk] = off;
     __syncthreads();
     const int tot = s_pre[kMergeChunk];
-    for (int f = threadIdx.x; f < tot; f += blockDim.x) {
-      int lo = 0, hi = kMergeChunk - 1;  // last buffer whose range starts at or before f
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_pre[mid] <= f) lo = mid; else hi = mid - 1;
+    // kBatch pairs per thread at a time: fixed-step searches for their
+    // buffers, then their loads, all independent, before the unions
+    constexpr int kBatch = 8;
+    for (int f0 = 0; f0 < tot; f0 += (int)blockDim.x * kBatch) {
+      int2 pr[kBatch];
+#pragma unroll
+      for (int j = 0; j < kBatch; ++j) {
+        const int f = f0 + j * (int)blockDim.x + (int)threadIdx.x;
+        // last buffer whose range starts at or before f (s_pre[0] = 0)
+        int lo = 0;
+#pragma unroll
+        for (int step = kMergeChunk / 2; step >= 1; step >>= 1)
+          if (s_pre[lo + step] <= f) lo += step;
+        pr[j] = f < tot ? d.pairs[(w0 + lo) * per + (f - s_pre[lo])] : make_int2(0, 0);
       }
-      const int2 pr = d.pairs[(w0 + lo) * per + (f - s_pre[lo])];
-      int32_t a = lds_find(par, pr.x), b = lds_find(par, pr.y);
-      while (a != b) {
-        if (a < b) {
-          int32_t tt = a;
-          a = b;
-          b = tt;
+#pragma unroll
+      for (int j = 0; j < kBatch; ++j) {
+        int32_t a = lds_find(par, pr[j].x), b = lds_find(par, pr[j].y);
+        while (a != b) {
+          if (a < b) {
+            int32_t tt = a;
+            a = b;
+            b = tt;
+          }
+          int32_t old = atomicCAS(&par[a], a, b);
+          if (old == a) break;
+          a = lds_find(par, old);
+          b = lds_find(par, b);
         }
-        int32_t old = atomicCAS(&par[a], a, b);
-        if (old == a) break;
-        a = lds_find(par, old);
-        b = lds_find(par, b);
       }
     }
     __syncthreads();  // s_pre is rewritten by the next chunk

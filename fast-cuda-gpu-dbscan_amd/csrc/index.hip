@@ -60,16 +60,18 @@ void launch_iota(int32_t* p, int64_t n, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ bbox --
-// slots per instance: 0..2 min (ordered keys, start at the largest key),
-// 3..5 max, 6 max |x|, 7 non-finite flag (all start at 0)
+// slots per instance and stripe: 0..2 min (ordered keys, start at the
+// largest key), 3..5 max, 6 max |x|, 7 non-finite flag (all start at 0)
 __global__ void k_bbox_init(Dev d) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)d.ninst * 8;
+  const int64_t n = (int64_t)d.ninst * kBoxStripes * 8;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     d.bbox[i] = (i & 7) < 3 ? ~0ull : 0ull;
 }
 
 void launch_bbox_init(const Dev& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_bbox_init, dim3(fill_grid((int64_t)d.ninst * 8)), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_bbox_init, dim3(fill_grid((int64_t)d.ninst * kBoxStripes * 8)),
+                     dim3(256), 0, s, d);
 }
 
 // One block per chunk (chunks never straddle instances).  Block reduction,
@@ -120,7 +122,8 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d) {
         s_red[0][3 + k] = fmax(s_red[0][3 + k], s_red[w][3 + k]);
         s_red[0][6] = fmax(s_red[0][6], s_red[w][6]);
       }
-    unsigned long long* bb = d.bbox + (int64_t)ch.inst * 8;
+    unsigned long long* bb =
+        d.bbox + ((int64_t)ch.inst * kBoxStripes + blockIdx.x % kBoxStripes) * 8;
     for (int k = 0; k < dim; ++k) {
       atomicMin(&bb[k], ord_d(s_red[0][k]));
       atomicMax(&bb[3 + k], ord_d(s_red[0][3 + k]));
@@ -175,7 +178,19 @@ __global__ void k_grid_setup(Dev d) {
     long long lines = 0;
     if (i < d.ninst) {
       Inst I = d.inst[i];
-      const unsigned long long* bb = d.bbox + (int64_t)i * 8;
+      unsigned long long bb[8];
+      {
+        const unsigned long long* bs = d.bbox + (int64_t)i * kBoxStripes * 8;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) bb[k] = k < 3 ? ~0ull : 0ull;
+#pragma unroll 4
+        for (int st = 0; st < kBoxStripes; ++st)
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const unsigned long long v = bs[st * 8 + k];
+            bb[k] = k < 3 ? (v < bb[k] ? v : bb[k]) : (v > bb[k] ? v : bb[k]);
+          }
+      }
       const int dim = I.dim;
       double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
       double emin = INFINITY;
@@ -726,31 +741,58 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
       }
     }
     // cells of each line with keys in [klo, khi] (at most 5): a line of at
-    // most 4 cells is read in one go, a longer one searched
-    uint64_t kk[5][4];
+    // most 4 cells is read in one go; the longer lines are searched in
+    // lockstep, so that the loads of the five searches overlap
+    int32_t lo[5], hi[5];
 #pragma unroll
-    for (int j = 0; j < 5; ++j)
+    for (int j = 0; j < 5; ++j) {
+      const bool srch = ok[j] && ls[j] >= 0 && le[j] - ls[j] > 4;
+      lo[j] = srch ? ls[j] : 0;
+      hi[j] = srch ? le[j] : 0;
+    }
+    for (;;) {
+      uint64_t km[5];
+      bool any = false;
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        kk[j][i] = (ok[j] && ls[j] >= 0 && le[j] - ls[j] <= 4 && ls[j] + i < le[j])
-                       ? d.cell_key[ls[j] + i]
-                       : ~0ull;
+      for (int j = 0; j < 5; ++j) {
+        km[j] = lo[j] < hi[j] ? d.cell_key[(lo[j] + hi[j]) >> 1] : 0;
+        any |= lo[j] < hi[j];
+      }
+      if (!any) break;
+#pragma unroll
+      for (int j = 0; j < 5; ++j)
+        if (lo[j] < hi[j]) {
+          const int32_t mid = (lo[j] + hi[j]) >> 1;
+          if (km[j] < klo[j])
+            lo[j] = mid + 1;
+          else
+            hi[j] = mid;
+        }
+    }
+    // first candidate of each line: the line start (short line) or the
+    // search result; the next (up to) 5 keys decide the run
+    uint64_t kk[5][5];
+    int32_t a0[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const bool live = ok[j] && ls[j] >= 0;
+      const bool shortl = live && le[j] - ls[j] <= 4;
+      a0[j] = shortl ? ls[j] : lo[j];
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+        kk[j][i] = (live && a0[j] + i < le[j] && (i < 4 || !shortl)) ? d.cell_key[a0[j] + i]
+                                                                     : ~0ull;
+    }
     int32_t packed[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
       packed[j] = 0;
       if (!ok[j] || ls[j] < 0) continue;
-      int32_t a = ls[j], b = ls[j];
-      if (le[j] - ls[j] <= 4) {
+      int32_t a = a0[j], b = a0[j];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          a += kk[j][i] < klo[j];
-          b += kk[j][i] <= khi[j];
-        }
-      } else {
-        a = lower_bound_key_in(d.cell_key, ls[j], le[j], klo[j]);
-        b = a;
-        for (int i = 0; i < 5 && a + i < le[j]; ++i) b += d.cell_key[a + i] <= khi[j];
+      for (int i = 0; i < 5; ++i) {
+        a += kk[j][i] < klo[j];
+        b += kk[j][i] <= khi[j];
       }
       packed[j] = (b > a) ? (a | ((b - a) << kRowCntShift)) : 0;
       if (d.nbr_stats) {

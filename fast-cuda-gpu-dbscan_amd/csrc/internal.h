@@ -161,6 +161,11 @@ struct SList {
 // order and the buffers are not swapped).
 constexpr int kMaxPasses = 8;
 
+// Bounding-box partials: chunk b reduces into stripe b % kBoxStripes of its
+// instance (so that the atomics of many chunks do not meet on one address);
+// the grid set-up reduces the stripes.
+constexpr int kBoxStripes = 32;
+
 // All device pointers of one run, passed to kernels by value.
 struct Dev {
   int64_t n;
@@ -169,7 +174,8 @@ struct Dev {
   const double* xyz;
   Inst* inst;
   const Chunk* chunks;
-  unsigned long long* bbox;  // [ninst][6] ordered-u64 min0..2 max0..2
+  unsigned long long* bbox;  // [ninst][kBoxStripes][8] ordered-u64 min0..2 max0..2,
+                             // max |x|, non-finite flag
   long long* ctl;
   long long* stat;  // [C_NCTL][kStripes] statistics counters (summed by the host)
 
