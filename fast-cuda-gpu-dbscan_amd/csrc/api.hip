@@ -100,7 +100,8 @@ enum Slot {
   S_DTOT, S_PASS, S_CSTART, S_CKEY, S_CINST, S_CM, S_ROWS, S_CELLOF, S_X0, S_X1, S_X2,
   S_G0, S_G1, S_G2, S_ORIG, S_CORE, S_LABEL, S_PARENT, S_COMPMIN, S_PAIRS, S_Q0, S_Q1,
   S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT, S_PAIRCNT, S_SEGPOS, S_LINES, S_AMASK,
-  S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP, S_ADJPTS, S_CORECELLS, S_TUNELAB, S_STAT, S_GLINES
+  S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP, S_ADJPTS, S_CORECELLS, S_TUNELAB, S_STAT, S_GLINES,
+  S_CZ
 };
 
 // line table entries (8 bytes each) above which neighbour lines are found
@@ -379,6 +380,7 @@ void finish_run(dclust_handle* h, const Dev& d, const RunShape& shape, long long
   stats->diag[12] = ctl_h[C_POPS_MAX];
   stats->diag[13] = ctl_h[C_SEEDS_MAX];
   stats->diag[14] = replayed ? 1 : 0;
+  stats->diag[15] = ctl_h[C_ROUNDS_MAX];
 }
 
 #define HIPCHK(x)                          \
@@ -481,6 +483,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     d.cell_start = h->get<int32_t>(S_CSTART, ntot + 1, &E);
     d.cell_key = h->get<uint64_t>(S_CKEY, ntot, &E);
     d.cell_xyz = h->get<uint64_t>(S_CXYZ, ntot, &E);
+    d.cell_z = h->get<int32_t>(S_CZ, ntot, &E);
     d.cell_comp = h->get<int32_t>(S_CCOMP, ntot, &E);
     // striped work lists: sub-list capacity = the most its workgroups append
     const int64_t cap_p = (point_blocks(ntot) + kStripes - 1) / kStripes * 256;

@@ -967,7 +967,7 @@ __global__ __launch_bounds__(256) void k_far(Dev d) {
 // input index of each component (the minimum over its chains).
 __global__ __launch_bounds__(256) void k_flatten_all(Dev d) {
   if (abandoned(d)) return;
-  const long long nc = d.ctl[C_NCHAIN];
+  const long long nc = chain_ids(d);
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc;
        c += (long long)gridDim.x * blockDim.x) {
     const int32_t r = uf_find(d.parent, (int32_t)c);
@@ -1020,7 +1020,7 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
   const int64_t seg_end = (int64_t)C * (gw + 1) / W;
   int left = budget;
   int sl = 0;  // seed-list length (wave-uniform)
-  long long pops = 0, ovf = 0, edges = 0, seeds = 0;
+  long long pops = 0, ovf = 0, edges = 0, seeds = 0, rounds = 0;
   for (;;) {
     if (left <= 0) {
       if (sl > 0) {
@@ -1055,7 +1055,9 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
           const int l = __ffsll((long long)m) - 1;
           int ok = 0;
           if (lane == 0) {
-            const int32_t id = (int32_t)atomicAdd((unsigned long long*)&d.ctl[C_NCHAIN], 1ull);
+            // a chain is named by its seed cell: unique, below the cell
+            // count, and no shared counter to contend on
+            const int32_t id = (int32_t)(seg + l);
             ok = atomicCAS(&d.cell_label[seg + l], kUnclaimed, id) == kUnclaimed;
           }
           ok = __shfl(ok, 0, 64);
@@ -1082,6 +1084,7 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
     sl -= P;
     left -= P;
     pops += P;
+    ++rounds;
     int32_t own = 0;
     unsigned long long m0 = 0;
     int32_t rA[9];  // the popped cell's near lines, loaded with its label and mask
@@ -1182,10 +1185,12 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
     atomicAdd((unsigned long long*)&d.ctl[C_POPS], (unsigned long long)pops);
     atomicMax((unsigned long long*)&d.ctl[C_POPS_MAX], (unsigned long long)pops);
     atomicMax((unsigned long long*)&d.ctl[C_SEEDS_MAX], (unsigned long long)seeds);
+    atomicMax((unsigned long long*)&d.ctl[C_ROUNDS_MAX], (unsigned long long)rounds);
     if (pops) atomicMax((unsigned long long*)&d.ctl[C_LASTPASS], (unsigned long long)(pass + 1));
   }
   stat_add(d, C_OVF, ovf);
   stat_add(d, C_EDGES, edges);
+  stat_add(d, C_NCHAIN, lane == 0 ? seeds : 0);  // seeds is wave-uniform
 }
 
 // After the last of a fixed number of expansion passes: work still pending
@@ -1231,10 +1236,12 @@ __global__ void k_seg_init(Dev d, int64_t W) {
 }
 
 // ----------------------------------------------------------------- merge --
-// When the chains fit in LDS (the usual case: a few thousand to a few tens
-// of thousands of chains), one workgroup merges every recorded pair on an
-// LDS copy of the chain forest with LDS atomics and writes the flattened
-// roots back.  Otherwise the pairs are merged in global memory.
+// Chain ids are seed cell indices.  When the id space fits in LDS (up to
+// 32768 cells), one workgroup merges every recorded pair on an LDS copy of
+// the chain forest with LDS atomics and writes the flattened roots back.
+// Otherwise the pairs are merged in global memory by the whole grid (the
+// ids are spread over the cell range, so the unions rarely share a cache
+// line).
 constexpr int kLdsChains = 32768;  // 128 KiB of LDS (gfx950 allows 160 KiB per workgroup)
 constexpr int kMergeChunk = 4096;  // pair buffers scanned at a time (16 KiB of LDS)
 
@@ -1256,7 +1263,7 @@ __global__ __launch_bounds__(1024) void k_merge_lds(Dev d) {
   __shared__ int32_t par[kLdsChains];
   __shared__ int32_t s_pre[kMergeChunk + 1];
   __shared__ int32_t s_wsum[16];
-  const long long nc = d.ctl[C_NCHAIN];
+  const long long nc = chain_ids(d);
   if (nc > kLdsChains) return;
   // parent may already hold inline merges (pairs past a full buffer)
   for (int i = threadIdx.x; i < nc; i += blockDim.x) par[i] = d.parent[i];
@@ -1333,7 +1340,7 @@ __global__ __launch_bounds__(1024) void k_merge_lds(Dev d) {
 
 __global__ __launch_bounds__(256) void k_union_pairs(Dev d) {
   if (abandoned(d)) return;
-  if (d.ctl[C_NCHAIN] <= kLdsChains) return;
+  if (chain_ids(d) <= kLdsChains) return;
   const int64_t nw = pair_waves(d);
   const int64_t wid = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nwv = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -1348,7 +1355,7 @@ __global__ __launch_bounds__(256) void k_union_pairs(Dev d) {
 
 __global__ __launch_bounds__(256) void k_flatten(Dev d) {
   if (abandoned(d)) return;
-  const long long nc = d.ctl[C_NCHAIN];
+  const long long nc = chain_ids(d);
   if (nc <= kLdsChains) return;
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc;
        c += (long long)gridDim.x * blockDim.x)

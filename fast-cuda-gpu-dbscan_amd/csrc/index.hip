@@ -608,7 +608,9 @@ __global__ __launch_bounds__(CS_T) void k_csr_write(Dev d) {
         d.cell_start[cid] = (int32_t)i;
         d.cell_key[cid] = k;
         d.cell_inst[cid] = inst;
-        d.cell_xyz[cid] = pack_cell(d.inst[inst], k);
+        const uint64_t pc = pack_cell(d.inst[inst], k);
+        d.cell_xyz[cid] = pc;
+        d.cell_z[cid] = (int32_t)(d.inst[inst].dim == 3 ? pc >> 42 : pc >> 32);
       }
       if (i == d.n - 1) {
         d.cell_start[cid + 1] = (int32_t)d.n;
@@ -642,24 +644,6 @@ __global__ __launch_bounds__(256) void k_gather(Dev d) {
 }
 
 // ------------------------------------------------------------------ rows --
-__device__ __forceinline__ int32_t lower_bound_key(const uint64_t* a, int32_t n, uint64_t v) {
-  int32_t lo = 0, hi = n;
-  while (lo < hi) {
-    int32_t mid = (lo + hi) >> 1;
-    if (a[mid] < v) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ int32_t lower_bound_key_in(const uint64_t* a, int32_t lo, int32_t hi,
-                                                      uint64_t v) {
-  while (lo < hi) {
-    int32_t mid = (lo + hi) >> 1;
-    if (a[mid] < v) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
 // line of a cell: all coordinates but the last; global line-table index
 __device__ __forceinline__ int64_t cell_line(const Dev& d, int32_t c) {
   const Inst& I = *inst_of_cell(d, c);
@@ -691,29 +675,38 @@ __global__ __launch_bounds__(256) void k_line_mark(Dev d) {
 
 // For each (cell, stencil line): the run of non-empty cells of that line
 // whose last coordinate is within +-2 of the cell's, packed as
-// first | count << 28 (count <= 5).  One thread per (cell, line).  The
-// neighbour line is found in the dense line table, or by binary search over
-// all cells when the table would be too large.
+// first | count << 28 (count <= 5).  The neighbour line's cell range comes
+// from the dense line table, or from two binary searches over all cell keys
+// when the table would be too large; inside the line the cells are found
+// by their last coordinate (cell_z), so the searches compare 32-bit values.
+__device__ __forceinline__ int32_t lower_bound_u64(const uint64_t* a, int32_t lo, int32_t hi,
+                                                   uint64_t v) {
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (a[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 __global__ __launch_bounds__(256) void k_rows(Dev d) {
   const int32_t C = (int32_t)d.ctl[C_NCELL];
   const bool table = d.ctl[C_NLINES] <= d.line_cap;
+  const int g = blockIdx.y;  // lines 5g .. 5g+4 of the stencil
   long long cand = 0, nbrc = 0;
-  // one lane per (group of 5 lines, cell), cells fastest: the lanes of a
-  // wave are consecutive cells, which mostly share their neighbour lines
-  // (broadcast loads), and each lane has five lines' loads in flight
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)C * 5;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int g = (int)(t / C);
-    const int32_t c = (int32_t)(t - (int64_t)g * C);
+  // one lane per cell, consecutive cells in consecutive lanes: they mostly
+  // share their neighbour lines (broadcast loads); each lane has five
+  // lines' loads in flight
+  for (int32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
     const Inst& I = *inst_of_cell(d, c);
     const int dim = I.dim;
     const int last = dim - 1;
     int32_t kc[3];
     unpack_cell(dim, d.cell_xyz[c], kc);
-    const int32_t lo_l = kc[last] >= 2 ? kc[last] - 2 : 0;
-    const int32_t hi_l = kc[last] + 2 < I.nk[last] ? kc[last] + 2 : I.nk[last] - 1;
-    bool ok[5];
-    uint64_t klo[5], khi[5];
+    const int32_t zlo = kc[last] >= 2 ? kc[last] - 2 : 0;
+    const int32_t zhi = kc[last] + 2 < I.nk[last] ? kc[last] + 2 : I.nk[last] - 1;
     int32_t ls[5], le[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
@@ -726,36 +719,36 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
         o0 = r < 5 ? kRowOff2[r] : 0;
       }
       const int32_t r0 = kc[0] + o0, r1 = kc[1] + o1;
-      ok[j] = (dim == 3 || g == 0) && r0 >= 0 && r0 < I.nk[0];
-      if (dim == 3) ok[j] = ok[j] && r1 >= 0 && r1 < I.nk[1];
+      bool ok = (dim == 3 || g == 0) && r0 >= 0 && r0 < I.nk[0];
+      if (dim == 3) ok = ok && r1 >= 0 && r1 < I.nk[1];
       const int64_t line = dim == 3 ? (int64_t)r0 * I.nk[1] + r1 : (int64_t)r0;
-      const int64_t base = line * I.nk[last];
-      klo[j] = (uint64_t)(I.key_base + base + lo_l);
-      khi[j] = (uint64_t)(I.key_base + base + hi_l);
-      ls[j] = 0;
-      le[j] = C;
-      if (ok[j] && table) {
+      ls[j] = -1;
+      le[j] = -1;
+      if (ok && table) {
         const int2 lr = d.lines[I.line_base + line];
         ls[j] = lr.x;
         le[j] = lr.y;
+      } else if (ok) {
+        const uint64_t k0 = (uint64_t)(I.key_base + line * I.nk[last]);
+        ls[j] = lower_bound_u64(d.cell_key, 0, C, k0);
+        le[j] = lower_bound_u64(d.cell_key, ls[j], C, k0 + (uint64_t)I.nk[last]);
       }
     }
-    // cells of each line with keys in [klo, khi] (at most 5): a line of at
-    // most 4 cells is read in one go; the longer lines are searched in
-    // lockstep, so that the loads of the five searches overlap
+    // lines of more than 4 cells: first cell with z >= zlo, the five
+    // searches in lockstep so that their loads overlap
     int32_t lo[5], hi[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
-      const bool srch = ok[j] && ls[j] >= 0 && le[j] - ls[j] > 4;
+      const bool srch = ls[j] >= 0 && le[j] - ls[j] > 4;
       lo[j] = srch ? ls[j] : 0;
       hi[j] = srch ? le[j] : 0;
     }
     for (;;) {
-      uint64_t km[5];
+      int32_t zm[5];
       bool any = false;
 #pragma unroll
       for (int j = 0; j < 5; ++j) {
-        km[j] = lo[j] < hi[j] ? d.cell_key[(lo[j] + hi[j]) >> 1] : 0;
+        zm[j] = lo[j] < hi[j] ? d.cell_z[(lo[j] + hi[j]) >> 1] : 0;
         any |= lo[j] < hi[j];
       }
       if (!any) break;
@@ -763,39 +756,37 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
       for (int j = 0; j < 5; ++j)
         if (lo[j] < hi[j]) {
           const int32_t mid = (lo[j] + hi[j]) >> 1;
-          if (km[j] < klo[j])
+          if (zm[j] < zlo)
             lo[j] = mid + 1;
           else
             hi[j] = mid;
         }
     }
-    // first candidate of each line: the line start (short line) or the
-    // search result; the next (up to) 5 keys decide the run
-    uint64_t kk[5][5];
+    // from the first candidate (line start of a short line, search result
+    // of a long one) the next up to 5 cells decide the run
+    int32_t zz[5][5];
     int32_t a0[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
-      const bool live = ok[j] && ls[j] >= 0;
-      const bool shortl = live && le[j] - ls[j] <= 4;
+      const bool live = ls[j] >= 0 && le[j] > ls[j];
+      const bool shortl = le[j] - ls[j] <= 4;
       a0[j] = shortl ? ls[j] : lo[j];
 #pragma unroll
       for (int i = 0; i < 5; ++i)
-        kk[j][i] = (live && a0[j] + i < le[j] && (i < 4 || !shortl)) ? d.cell_key[a0[j] + i]
-                                                                     : ~0ull;
+        zz[j][i] = (live && a0[j] + i < le[j] && (i < 4 || !shortl)) ? d.cell_z[a0[j] + i]
+                                                                     : 0x7fffffff;
     }
     int32_t packed[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
-      packed[j] = 0;
-      if (!ok[j] || ls[j] < 0) continue;
       int32_t a = a0[j], b = a0[j];
 #pragma unroll
       for (int i = 0; i < 5; ++i) {
-        a += kk[j][i] < klo[j];
-        b += kk[j][i] <= khi[j];
+        a += zz[j][i] < zlo;
+        b += zz[j][i] <= zhi;
       }
-      packed[j] = (b > a) ? (a | ((b - a) << kRowCntShift)) : 0;
-      if (d.nbr_stats) {
+      packed[j] = (ls[j] >= 0 && b > a) ? (a | ((b - a) << kRowCntShift)) : 0;
+      if (d.nbr_stats && packed[j]) {
         nbrc += b - a;
         cand += (long long)(d.cell_start[c + 1] - d.cell_start[c]) *
                 (d.cell_start[b] - d.cell_start[a]);
@@ -832,7 +823,7 @@ void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
   hipLaunchKernelGGL(k_gather, dim3(fill_grid(n)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_line_clear, dim3(grid_cells), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_line_mark, dim3(grid_cells), dim3(256), 0, s, d);
-  hipLaunchKernelGGL(k_rows, dim3(grid_cells * 4), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_rows, dim3(grid_cells, 5), dim3(256), 0, s, d);
 }
 
 void launch_scan_top(int64_t* v, int nb, hipStream_t s) {

@@ -98,7 +98,7 @@ struct Chunk {
 enum Ctl : int {
   C_KEYBITS = 0,
   C_NCELL,
-  C_NCHAIN,
+  C_NCHAIN,     // chains started (a chain id is its seed cell's index)
   C_NPAIR,
   C_Q0_N,
   C_Q1_N,
@@ -138,9 +138,10 @@ enum Ctl : int {
   C_ABANDON,    // expansion left work pending in a run with a fixed pass
                 // count: the kernels after it return at once
   C_IDLE,       // nothing pending for the current expansion pass
+  C_ROUNDS_MAX, // most pop rounds of one expansion wave in one pass
   C_NCTL = 48
 };
-static_assert(C_IDLE < C_NCTL, "control block too small");
+static_assert(C_ROUNDS_MAX < C_NCTL, "control block too small");
 
 // Statistics counters are striped: a wave adds to one of kStripes copies.
 constexpr int kStripes = 64;
@@ -191,6 +192,7 @@ struct Dev {
   int32_t* cell_start;  // [N+1]
   uint64_t* cell_key;   // [N]
   uint64_t* cell_xyz;   // [N] packed integer cell coordinates
+  int32_t* cell_z;      // [N] last integer cell coordinate (position in its line)
   int32_t* cell_inst;   // [N]
   int32_t* cell_ncore;  // [N] core points per cell
   int32_t* cell_label;  // [N] chain owning the cell's core points
@@ -314,6 +316,9 @@ __device__ __forceinline__ void wave_add(long long* p, long long v) {
 // of the counter, picked by the wave's position in the launch, so that a
 // launch of many waves does not serialise on one address.  The host sums
 // the copies.
+// chain ids are seed cell indices: the id space is the cell count
+__device__ __forceinline__ long long chain_ids(const Dev& d) { return d.ctl[C_NCELL]; }
+
 // set when a run with a fixed number of expansion passes left work pending
 __device__ __forceinline__ bool abandoned(const Dev& d) { return d.ctl[C_ABANDON] != 0; }
 

@@ -117,7 +117,8 @@ typedef struct dclust_stats {
    * pair tests of the far-pair pass, [11] far pairs that joined two
    * components, [12] most cells popped by one expansion wave in a pass,
    * [13] most chains started by one expansion wave in a pass, [14] 1 if
-   * the run was a replay of a captured graph, [15] reserved */
+   * the run was a replay of a captured graph, [15] most pop rounds (up to
+   * 64 cells each) of one expansion wavefront in a pass */
   int64_t diag[16];
 } dclust_stats;
 

@@ -28,7 +28,10 @@ def _check(handle, x, eps, minpts, prec, **kw):
     nbad = int(np.sum(lab != ref))
     assert nbad_core == 0, f"{nbad_core} core flags differ"
     assert nbad == 0, f"{nbad} labels differ"
-    return handle.last_stats
+    st = handle.last_stats
+    # every cluster holds at least one chain; a chain starts at a core cell
+    assert st["n_clusters"] <= st["n_chains"] <= st["n_core"]
+    return st
 
 
 @pytest.mark.parametrize("prec", PRECS)
@@ -336,3 +339,16 @@ def test_graph_replay_non_finite_input(handle, devmem, monkeypatch):
         assert np.array_equal(lab, ref), f"call {step}"
         seen.append(st["diag"][14])
     assert seen == [0, 1, 1]
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_no_line_table(handle, prec):
+    # 2-D points spread over more than 2^25 cells along x: the dense line
+    # table is not built and neighbour lines are found by binary search
+    rng = np.random.default_rng(21)
+    n = 6000
+    centres = np.array([[0.0, 0.0], [40.0, 3.0], [1.8e7, -5.0], [1.8e7 + 30.0, 0.0]])
+    x = centres[rng.integers(0, 4, n)] + rng.normal(0, 2.0, (n, 2))
+    x[:300, 0] = rng.uniform(0, 1.8e7, 300)
+    st = _check(handle, x, 0.6, 4, prec)
+    assert st["n_cells"] > 1000
