@@ -62,11 +62,11 @@ constexpr int kQCache = 256;  // query points a heavy-cell wavefront keeps in LD
 
 template <bool F32>
 struct Batch {
-  static constexpr int value = F32 ? 8 : 4;  // candidates loaded per lane step
+  static constexpr int value = F32 ? 6 : 4;  // candidates loaded per lane step
 };
 
-struct Tally {
-  long long band = 0, flip = 0, asym = 0, eval = 0;
+struct Tally {  // per-lane event counts (summed per wave by stat_add)
+  int band = 0, flip = 0, asym = 0, eval = 0;
 };
 
 // ------------------------------------------------------------ predicates --
@@ -281,7 +281,7 @@ __device__ __forceinline__ void block_append(bool take, int32_t value, const SLi
 // pairs).
 template <int D, bool F32>
 __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int32_t qs,
-                          int32_t qe, int R, Tally& t, bool& isc) {
+                          int32_t qe, int R, int32_t* rk, Tally& t, bool& isc) {
   constexpr int NR = D == 3 ? 25 : 5;
   constexpr int B = Batch<F32>::value;
   int cnt = qe - qs;  // the own cell is a clique, p itself included
@@ -301,12 +301,10 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
       hi[k] = px[k] + P.eps;
     }
   }
-  // records stay in registers until the point is known to be non-core
-  int32_t rk[kRecSlots];
-#pragma unroll
-  for (int i = 0; i < kRecSlots; ++i) rk[i] = 0;
+  // records stay in this lane's LDS slots until the point is known to be
+  // non-core
   int nrec = 0;
-  long long ev = 0;
+  int ev = 0;
   const RowsOf rows = rows_of(d, c);
   for (int g = 0; g < NR && cnt < P.minpts; g += 5) {
     int32_t aa[5], ll[5];
@@ -357,8 +355,7 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
           }
           cnt += fwd;
           if (sym) {
-#pragma unroll
-            for (int i = 0; i < kRecSlots; ++i) rk[i] = nrec == i ? q : rk[i];
+            if (nrec < kRecSlots) rk[nrec] = q;
             ++nrec;
           }
           ++ev;
@@ -369,9 +366,7 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
   isc = cnt >= P.minpts;
   if (!isc) {
     int32_t* rec = d.rec + (int64_t)p * R;
-#pragma unroll
-    for (int i = 0; i < kRecSlots; ++i)
-      if (i < nrec && i < R) rec[i] = rk[i];
+    for (int i = 0; i < nrec && i < R; ++i) rec[i] = rk[i];
     d.rec_n[p] = (uint8_t)(nrec < 255 ? nrec : 255);
   }
   t.eval += ev;
@@ -397,8 +392,11 @@ __global__ __launch_bounds__(256) void k_core(Dev d) {
   bool isc = false;
   const int R = d.rec_stride;
   const Par P(*I);
-  DCLUST_BY_DIM(act, dim, (core_lane<3, F32>(d, P, p, c, qs, qe, R, t, isc)),
-                (core_lane<2, F32>(d, P, p, c, qs, qe, R, t, isc)));
+  // kRecSlots record slots per lane (stride kRecSlots + 1: no bank conflicts)
+  __shared__ int32_t s_rec[256 * (kRecSlots + 1)];
+  int32_t* rk = s_rec + threadIdx.x * (kRecSlots + 1);
+  DCLUST_BY_DIM(act, dim, (core_lane<3, F32>(d, P, p, c, qs, qe, R, rk, t, isc)),
+                (core_lane<2, F32>(d, P, p, c, qs, qe, R, rk, t, isc)));
   if (act) d.core[p] = isc;
   // core points of light cells: the lanes of the near-adjacency pass
   block_append(isc && qe - qs <= kHeavy, p, d.adj_pts);
@@ -518,14 +516,14 @@ __device__ bool cells_joined(const Dev& d, const Par& P, int32_t qs, int32_t qe,
 }
 
 // Offset along the last axis of cell B (in stencil line r of cell A).
+// (B lies in a neighbour line of A, so the difference of their last
+// coordinates is the offset.)
 template <int D>
 __device__ __forceinline__ int cell_offset_last(const Dev& d, const Inst& I, int32_t A, int32_t B,
                                                 int r) {
-  const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
-  const int o1 = D == 3 ? kRowOff3[r][1] : 0;
-  const int64_t nkl = I.nk[D - 1];
-  const int64_t line_delta = D == 3 ? ((int64_t)o0 * I.nk[1] + o1) * nkl : (int64_t)o0 * nkl;
-  return (int)((int64_t)d.cell_key[B] - (int64_t)d.cell_key[A] - line_delta);
+  (void)I;
+  (void)r;
+  return d.cell_z[B] - d.cell_z[A];
 }
 
 // Near adjacency of light cells: one lane per core point p of a light cell
@@ -555,28 +553,32 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
   for (int k = 0; k < D; ++k) px[k] = (!F32 && act) ? d.X[k][p] : 0.0;
 #pragma unroll
   for (int k = 0; k < D; ++k) fr[k] = (int32_t)(pg[k] & kFrac);
-  const int64_t keyA = act ? (int64_t)d.cell_key[c] : 0;
-  const int64_t nkl = I.nk[D - 1];
-  const int64_t nk1 = I.nk[1];
-  const RowsOf rows = rows_of(d, c);
+  const int32_t zA = act ? d.cell_z[c] : 0;
+  // the near lines of the cell, loaded together
+  int32_t rn[NN];
+  {
+    const RowsOf rows = rows_of(d, c);
+#pragma unroll
+    for (int r = 0; r < NN; ++r) rn[r] = act ? rows[r] : 0;
+  }
   for (int step = 0; step < NN; ++step) {
     const int r = (step + rank) % NN;
     const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
     const int o1 = D == 3 ? kRowOff3[r][1] : 0;
-    const int32_t pk = act ? rows[r] : 0;
+    int32_t pk = rn[0];
+#pragma unroll
+    for (int i = 1; i < NN; ++i) pk = r == i ? rn[i] : pk;
     const int32_t len = pk >> kRowCntShift;
     if (len > 0) {
       const int32_t st = pk & kRowCellMask;
-      int32_t bnd[6], nco[5];
-      int64_t key[5];
+      int32_t bnd[6], nco[5], zb[5];
 #pragma unroll
       for (int k = 0; k < 6; ++k) bnd[k] = k <= len ? d.cell_start[st + k] : 0;
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
-        key[k] = k < len ? (int64_t)d.cell_key[st + k] : 0;
+        zb[k] = k < len ? d.cell_z[st + k] : 0;
         nco[k] = k < len ? d.cell_ncore[st + k] : 0;
       }
-      const int64_t line_delta = D == 3 ? ((int64_t)o0 * nk1 + o1) * nkl : (int64_t)o0 * nkl;
       float g2 = 0.f, h2 = 0.f;  // over the line's axes
       {
         const float a = cell_gap(o0, fr[0]), fa = cell_far(o0, fr[0]);
@@ -593,7 +595,7 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
         const int32_t n = k < len ? bnd[k + 1] - bnd[k] : 0;
-        const int ol = (int)(key[k] - keyA - line_delta);  // offset along the last axis
+        const int ol = zb[k] - zA;  // offset along the last axis
         const float e = cell_gap(ol, fr[D - 1]), fe = cell_far(ol, fr[D - 1]);
         const bool reach = !(__builtin_fmaf(e, e, g2) > P.d2_out);
         const bool inside = __builtin_fmaf(fe, fe, h2) < P.d2_in;
@@ -747,19 +749,14 @@ __device__ void adj_heavy(const Dev& d, const Inst& I, const Par& P, int32_t A, 
   const bool cached = nq <= kQCache;
   const int NQ = cached ? nq : (qe - qs);
   const int32_t pkl = lane < NN ? rows_of(d, A)[lane] : 0;
-  const int64_t keyA = (int64_t)d.cell_key[A];
-  const int64_t nkl = I.nk[D - 1];
   unsigned long long m0 = 0;
   for (int r = 0; r < NN; ++r) {
     const int32_t pk = rl_i(pkl, r);
     const int32_t len = pk >> kRowCntShift, st = pk & kRowCellMask;
-    const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
-    const int o1 = D == 3 ? kRowOff3[r][1] : 0;
-    const int64_t line_delta = D == 3 ? ((int64_t)o0 * I.nk[1] + o1) * nkl : (int64_t)o0 * nkl;
     for (int k = 0; k < len; ++k) {
       const int32_t Bc = st + k;
       if (Bc == A) continue;
-      const int ol = (int)((int64_t)d.cell_key[Bc] - keyA - line_delta);
+      const int ol = cell_offset_last<D>(d, I, A, Bc, r);
       if (ol < -1 || ol > 1) continue;  // a far cell
       const int32_t b0 = d.cell_start[Bc], b1 = d.cell_start[Bc + 1];
       if (d.cell_ncore[Bc] == 0) continue;
@@ -891,8 +888,6 @@ __device__ long long far_cell(const Dev& d, const Inst& I, const Par& P, int32_t
   const int32_t qs = d.cell_start[A], qe = d.cell_start[A + 1];
   const bool allc = d.cell_ncore[A] == qe - qs;
   const int32_t compA = d.cell_comp[A];
-  const int64_t keyA = (int64_t)d.cell_key[A];
-  const int64_t nkl = I.nk[D - 1];
   const RowsOf rows = rows_of(d, A);
   long long joins = 0;
   for (int r = 0; r < NR; ++r) {
@@ -904,7 +899,6 @@ __device__ long long far_cell(const Dev& d, const Inst& I, const Par& P, int32_t
     const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
     const int o1 = D == 3 ? kRowOff3[r][1] : 0;
     const bool far_l = o0 == 2 || o0 == -2 || o1 == 2 || o1 == -2;
-    const int64_t line_delta = D == 3 ? ((int64_t)o0 * I.nk[1] + o1) * nkl : (int64_t)o0 * nkl;
     // cells above A in another component (cell_comp is -1 for a cell
     // without core points); in a near line only the cells two steps away
     // along the line are far
@@ -920,7 +914,7 @@ __device__ long long far_cell(const Dev& d, const Inst& I, const Par& P, int32_t
 #pragma unroll
       for (int k = 0; k < 5; ++k)
         if (todo & (1u << k)) {
-          const int ol = (int)((int64_t)d.cell_key[st + k] - keyA - line_delta);
+          const int ol = cell_offset_last<D>(d, I, A, st + k, r);
           if (ol != 2 && ol != -2) todo &= ~(1u << k);
         }
     }
