@@ -988,6 +988,7 @@ constexpr int kSeen = 256;   // per-wave table of recently recorded collision pa
 __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budget) {
   extern __shared__ int32_t s_lists[];
   __shared__ unsigned long long s_seen[4][kSeen];
+  __shared__ uint8_t s_tag[4][kSeen];  // per-wave dedup slots of one step
   if (d.ctl[C_IDLE]) return;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   const int nwb = blockDim.x >> 6;
@@ -996,6 +997,7 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
   const int32_t C = (int32_t)d.ctl[C_NCELL];
   int32_t* list = s_lists + wv * S;
   unsigned long long* seen = s_seen[wv];
+  uint8_t* tag = s_tag[wv];
   int32_t* qout = d.q[(pass + 1) & 1];
   unsigned long long* qout_n = (unsigned long long*)&d.ctl[C_Q0_N + ((pass + 1) & 1)];
   const int32_t* qin = d.q[pass & 1];
@@ -1090,8 +1092,9 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
     }
     {
       const RowsOf rows = rows_of(d, A);
+      // loaded together with the label and mask (no wait for the mask)
 #pragma unroll
-      for (int r = 0; r < 9; ++r) rA[r] = (act && m0) ? rows[r] : 0;
+      for (int r = 0; r < 9; ++r) rA[r] = act ? rows[r] : 0;
     }
     edges += __popcll(m0);
     while (ballot(m0 != 0ull)) {
@@ -1142,26 +1145,34 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
           sl += __popcll(mc);
           sl = sl < S ? sl : S;
         }
-        // collisions: each distinct (own, other) pair of the step once
-        bool coll = v[j] && lab[j] != own;
-        unsigned long long cm = ballot(coll);
-        while (cm) {
-          const int l = __ffsll((long long)cm) - 1;
-          const int32_t o = rl_i(own, l), f = rl_i(lab[j], l);
-          const unsigned long long key = ((unsigned long long)(uint32_t)o << 32) | (uint32_t)f;
-          const int h = (int)(((uint32_t)o * 0x9E3779B1u ^ (uint32_t)f * 0x85EBCA6Bu) >> 24);
-          if (seen[h] != key) {
-            if (lane == 0) {
-              seen[h] = key;
-              if (pc < pcap)
-                pbuf[pc] = make_int2(o, f);
-              else
-                uf_union(d.parent, o, f);  // buffer full: merge inline
-            }
-            ++pc;
+        // collisions, recorded by all lanes at once: a pair already in the
+        // wave's table of recent pairs is skipped; among the lanes of this
+        // step with the same pair, the one whose id survives in the tag slot
+        // of the pair's hash records it (a lane whose hash slot was won by a
+        // different pair records its own; a rare duplicate is harmless)
+        const bool coll = v[j] && lab[j] != own;
+        if (ballot(coll)) {
+          const unsigned long long key =
+              ((unsigned long long)(uint32_t)own << 32) | (uint32_t)lab[j];
+          const int h =
+              (int)(((uint32_t)own * 0x9E3779B1u ^ (uint32_t)lab[j] * 0x85EBCA6Bu) >> 24);
+          bool nw = coll && seen[h] != key;
+          if (nw) tag[h] = (uint8_t)lane;
+          wave_sync();
+          const int w = nw ? (int)tag[h] : lane;
+          const uint32_t wlo = (uint32_t)__shfl((int)(uint32_t)key, w, 64);
+          const uint32_t whi = (uint32_t)__shfl((int)(uint32_t)(key >> 32), w, 64);
+          if (nw && w != lane && (((unsigned long long)whi << 32) | wlo) == key) nw = false;
+          const unsigned long long mn = ballot(nw);
+          if (nw) {
+            seen[h] = key;
+            const int slot = pc + (int)mbcnt(mn);
+            if (slot < pcap)
+              pbuf[slot] = make_int2(own, lab[j]);
+            else
+              uf_union(d.parent, own, lab[j]);  // buffer full: merge inline
           }
-          coll = coll && !(own == o && lab[j] == f);
-          cm = ballot(coll);
+          pc += __popcll(mn);
         }
       }
     }
