@@ -985,10 +985,19 @@ __global__ __launch_bounds__(256) void k_flatten_all(Dev d) {
 // reaches the end of the kernel: there is no inter-wave waiting.
 constexpr int kSeen = 256;   // per-wave table of recently recorded collision pairs
 
+// The cells one wave popped in a round (one per lane).
+struct PopStage {
+  unsigned long long mask[64];  // near adjacency bits
+  int32_t rows[9][64];          // near lines
+  int32_t own[64];              // chain
+  int32_t pre[64];              // exclusive prefix of the mask bit counts
+};
+
 __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budget) {
   extern __shared__ int32_t s_lists[];
   __shared__ unsigned long long s_seen[4][kSeen];
   __shared__ uint8_t s_tag[4][kSeen];  // per-wave dedup slots of one step
+  __shared__ PopStage s_pop[4];         // per-wave popped cells of a round
   if (d.ctl[C_IDLE]) return;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   const int nwb = blockDim.x >> 6;
@@ -998,6 +1007,7 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
   int32_t* list = s_lists + wv * S;
   unsigned long long* seen = s_seen[wv];
   uint8_t* tag = s_tag[wv];
+  PopStage& pst = s_pop[wv];
   int32_t* qout = d.q[(pass + 1) & 1];
   unsigned long long* qout_n = (unsigned long long*)&d.ctl[C_Q0_N + ((pass + 1) & 1)];
   const int32_t* qin = d.q[pass & 1];
@@ -1083,48 +1093,73 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
     ++rounds;
     int32_t own = 0;
     unsigned long long m0 = 0;
-    int32_t rA[9];  // the popped cell's near lines, loaded with its label and mask
     if (act) {
       // coherent read: the claim may have been made moments ago on another
       // XCD, whose L2 is not coherent with this one for plain loads
       own = __hip_atomic_load(&d.cell_label[A], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       m0 = d.amask[A];
     }
+    // the popped cells' chains, masks and near lines go to LDS (the lines
+    // loaded together with the label and mask), and their adjacency bits are
+    // dealt out evenly: item t of the round is bit k of the mask of lane
+    // src, where src is the last lane whose bit prefix is at most t; a round
+    // then takes ceil(bits / (64 T)) steps instead of the most bits of one
+    // lane / T
     {
       const RowsOf rows = rows_of(d, A);
-      // loaded together with the label and mask (no wait for the mask)
 #pragma unroll
-      for (int r = 0; r < 9; ++r) rA[r] = act ? rows[r] : 0;
+      for (int r = 0; r < 9; ++r) pst.rows[r][lane] = act ? rows[r] : 0;
     }
-    edges += __popcll(m0);
-    while (ballot(m0 != 0ull)) {
-      constexpr int T = 8;  // neighbours per lane step
-      int32_t tg[T];
+    const int nb = __popcll(m0);
+    edges += nb;
+    const int incl = wave_incl_scan(nb);
+    const int total = __shfl(incl, 63, 64);
+    pst.own[lane] = own;
+    pst.mask[lane] = m0;
+    pst.pre[lane] = incl - nb;
+    wave_sync();
+    constexpr int T = 8;  // items per lane step
+    for (int base = 0; base < total; base += 64 * T) {
+      int32_t tg[T], ow[T];
       bool v[T];
 #pragma unroll
       for (int j = 0; j < T; ++j) {
-        v[j] = m0 != 0ull;
-        const int b = v[j] ? __ffsll((long long)m0) - 1 : 0;
-        m0 &= m0 - 1;
-        const int r = (b * 205) >> 10;  // b / 5 for 0 <= b < 128
-        int32_t first = rA[0];
+        const int t = base + j * 64 + lane;
+        v[j] = t < total;
+        int src = 0;
 #pragma unroll
-        for (int i = 1; i < 9; ++i) first = r == i ? rA[i] : first;
-        tg[j] = v[j] ? (first & kRowCellMask) + (b - 5 * r) : 0;
+        for (int st = 32; st >= 1; st >>= 1)
+          if (pst.pre[src + st] <= t) src += st;
+        // bit k of the source mask: its k-th set bit
+        int k = t - pst.pre[src];
+        unsigned long long m = pst.mask[src];
+        int b = 0;
+#pragma unroll
+        for (int w = 32; w >= 1; w >>= 1) {
+          const int c = __popcll(m & ((1ull << w) - 1ull));
+          if (k >= c) {
+            k -= c;
+            m >>= w;
+            b += w;
+          }
+        }
+        const int r = (b * 205) >> 10;  // b / 5 for 0 <= b < 128
+        tg[j] = v[j] ? (pst.rows[r < 9 ? r : 0][src] & kRowCellMask) + (b - 5 * r) : 0;
+        ow[j] = pst.own[src];
       }
       // a plain load can only be stale as kUnclaimed (a label is written
       // once), so only those cells pay for a CAS
       int32_t lab[T];
 #pragma unroll
-      for (int j = 0; j < T; ++j) lab[j] = v[j] ? d.cell_label[tg[j]] : own;
+      for (int j = 0; j < T; ++j) lab[j] = v[j] ? d.cell_label[tg[j]] : ow[j];
       bool cl[T];
 #pragma unroll
       for (int j = 0; j < T; ++j) {
         cl[j] = false;
         if (v[j] && lab[j] == kUnclaimed) {
-          const int32_t old = atomicCAS(&d.cell_label[tg[j]], kUnclaimed, own);
+          const int32_t old = atomicCAS(&d.cell_label[tg[j]], kUnclaimed, ow[j]);
           cl[j] = old == kUnclaimed;
-          lab[j] = cl[j] ? own : old;
+          lab[j] = cl[j] ? ow[j] : old;
         }
       }
 #pragma unroll
@@ -1150,12 +1185,12 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
         // step with the same pair, the one whose id survives in the tag slot
         // of the pair's hash records it (a lane whose hash slot was won by a
         // different pair records its own; a rare duplicate is harmless)
-        const bool coll = v[j] && lab[j] != own;
+        const bool coll = v[j] && lab[j] != ow[j];
         if (ballot(coll)) {
           const unsigned long long key =
-              ((unsigned long long)(uint32_t)own << 32) | (uint32_t)lab[j];
+              ((unsigned long long)(uint32_t)ow[j] << 32) | (uint32_t)lab[j];
           const int h =
-              (int)(((uint32_t)own * 0x9E3779B1u ^ (uint32_t)lab[j] * 0x85EBCA6Bu) >> 24);
+              (int)(((uint32_t)ow[j] * 0x9E3779B1u ^ (uint32_t)lab[j] * 0x85EBCA6Bu) >> 24);
           bool nw = coll && seen[h] != key;
           if (nw) tag[h] = (uint8_t)lane;
           wave_sync();
@@ -1168,9 +1203,9 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
             seen[h] = key;
             const int slot = pc + (int)mbcnt(mn);
             if (slot < pcap)
-              pbuf[slot] = make_int2(own, lab[j]);
+              pbuf[slot] = make_int2(ow[j], lab[j]);
             else
-              uf_union(d.parent, own, lab[j]);  // buffer full: merge inline
+              uf_union(d.parent, ow[j], lab[j]);  // buffer full: merge inline
           }
           pc += __popcll(mn);
         }
