@@ -206,6 +206,15 @@ def batched_leg(hip, h, steps, warmup, check_oracle=True, count=4096):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle  # checker only, untimed
 
+        # instances where the two runs disagree: which one is wrong
+        diff = []
+        for i in bad[:8]:
+            ref = oracle.fast_dbscan(xs[i], infos[i]["eps"], infos[i]["minpts"])[0]
+            s = slice(off_pt[i], off_pt[i + 1])
+            diff.append({"instance": i, "n": int(ns[i]), "dim": int(dims[i]),
+                         "batched_vs_oracle": int(np.sum(lab_b[s] != ref)),
+                         "per_instance_vs_oracle": int(np.sum(lab_1[s] != ref))})
+        out["differing_detail"] = diff
         order = np.argsort(ns)
         sample = sorted(set(order[np.linspace(0, count - 1, 24).astype(int)].tolist()))
         mism = 0

@@ -964,7 +964,7 @@ __global__ __launch_bounds__(256) void k_flatten_all(Dev d) {
   const long long nc = chain_ids(d);
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc;
        c += (long long)gridDim.x * blockDim.x) {
-    const int32_t r = uf_find(d.parent, (int32_t)c);
+    const int32_t r = uf_root(d.parent, (int32_t)c);
     d.parent[c] = r;
     const int32_t m = d.compmin[c];
     if (m != kNoKey && r != c) atomicMin(&d.compmin[r], m);
@@ -1399,7 +1399,7 @@ __global__ __launch_bounds__(256) void k_flatten(Dev d) {
   if (nc <= kLdsChains) return;
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc;
        c += (long long)gridDim.x * blockDim.x)
-    d.parent[c] = uf_find(d.parent, (int32_t)c);
+    d.parent[c] = uf_root(d.parent, (int32_t)c);
 }
 
 // chain of a core point: the chain that claimed its cell

@@ -352,3 +352,49 @@ def test_no_line_table(handle, prec):
     x[:300, 0] = rng.uniform(0, 1.8e7, 300)
     st = _check(handle, x, 0.6, 4, prec)
     assert st["n_cells"] > 1000
+
+
+def test_back_to_back_runs_stable(devmem):
+    # a race in the final union-find flatten once left inner nodes as
+    # "roots" (a few points took another cluster's label, in about one of
+    # four sweeps over these instances); it showed in long sequences of
+    # different instances on one handle.  The 4096 batch instances are
+    # clustered back to back three times: every sweep must give the same
+    # labels, and a sample equals the oracle.
+    from concurrent.futures import ThreadPoolExecutor
+
+    import dclust as dc
+
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        insts = list(ex.map(dc.synth_batch_instance, range(4096)))
+    big = list(range(len(insts)))
+    xs = [insts[i][0] for i in big]
+    infos = [insts[i][1] for i in big]
+    ns = np.array([x.shape[0] for x in xs], dtype=np.int64)
+    flat = np.concatenate([x.ravel() for x in xs])
+    off_el = np.concatenate([[0], np.cumsum(ns * np.array([x.shape[1] for x in xs]))])
+    off_pt = np.concatenate([[0], np.cumsum(ns)])
+    d_x = devmem.upload(flat)
+    d_l = devmem.alloc(int(ns.sum()) * 4)
+    params = [dc.make_params(x.shape[1], f["eps"], f["minpts"], precision=dc.FP32_GUARD)
+              for x, f in zip(xs, infos)]
+    h = dc.Handle(0)
+    try:
+        first = None
+        for sweep in range(3):
+            for k in range(len(xs)):
+                h.run_device(d_x + int(off_el[k]) * 8, int(ns[k]), params[k],
+                             d_l + int(off_pt[k]) * 4)
+            lab = devmem.download(np.empty(int(ns.sum()), dtype=np.int32), d_l)
+            if first is None:
+                first = lab
+                for k in range(0, len(xs), 256):
+                    ref = oracle.fast_dbscan(xs[k], infos[k]["eps"], infos[k]["minpts"])[0]
+                    assert np.array_equal(lab[off_pt[k]:off_pt[k + 1]], ref), f"instance {big[k]}"
+            else:
+                diff = [big[k] for k in range(len(xs))
+                        if not np.array_equal(lab[off_pt[k]:off_pt[k + 1]],
+                                              first[off_pt[k]:off_pt[k + 1]])]
+                assert not diff, f"sweep {sweep}: instances {diff} changed"
+    finally:
+        h.close()
