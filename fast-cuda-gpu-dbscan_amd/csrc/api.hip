@@ -106,7 +106,7 @@ enum Slot {
 
 // line table entries (8 bytes each) above which neighbour lines are found
 // by binary search instead
-constexpr long long kMaxLineTable = 1LL << 25;
+constexpr long long kMaxLineTable = 1LL << 27;  // 1 GiB at most
 // expansion passes a captured run may hold; runs that need more stay eager
 constexpr int kGraphMaxPasses = 8;
 

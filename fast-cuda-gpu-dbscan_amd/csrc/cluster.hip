@@ -957,9 +957,21 @@ __global__ __launch_bounds__(256) void k_far(Dev d) {
   stat_add(d, C_FAR_JOINS, joins);
 }
 
-// Roots of every chain after the far-pair unions, and the minimum core
-// input index of each component (the minimum over its chains).
+// Roots of every chain after the far-pair unions, in two passes.  The first
+// compresses the forest with path-halving walks; a concurrent walk may
+// leave an entry at an inner ancestor instead of the root, so the second
+// pass walks the (now shallow) forest without writing anything but each
+// chain's own root, and moves each chain's minimum core input index to its
+// root: every entry is then exactly its root.
 __global__ __launch_bounds__(256) void k_flatten_all(Dev d) {
+  if (abandoned(d)) return;
+  const long long nc = chain_ids(d);
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc;
+       c += (long long)gridDim.x * blockDim.x)
+    d.parent[c] = uf_find(d.parent, (int32_t)c);
+}
+
+__global__ __launch_bounds__(256) void k_roots_final(Dev d) {
   if (abandoned(d)) return;
   const long long nc = chain_ids(d);
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc;
@@ -1399,7 +1411,7 @@ __global__ __launch_bounds__(256) void k_flatten(Dev d) {
   if (nc <= kLdsChains) return;
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc;
        c += (long long)gridDim.x * blockDim.x)
-    d.parent[c] = uf_root(d.parent, (int32_t)c);
+    d.parent[c] = uf_find(d.parent, (int32_t)c);
 }
 
 // chain of a core point: the chain that claimed its cell
@@ -1661,6 +1673,7 @@ void launch_far(const Dev& d, bool f32, int grid, hipStream_t s) {
   else
     hipLaunchKernelGGL(k_far<false>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_flatten_all, dim3(grid), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_roots_final, dim3(grid), dim3(256), 0, s, d);
 }
 
 void launch_border(const Dev& d, bool f32, hipStream_t s) {

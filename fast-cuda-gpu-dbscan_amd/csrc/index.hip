@@ -679,16 +679,29 @@ __global__ __launch_bounds__(256) void k_line_mark(Dev d) {
 // from the dense line table, or from two binary searches over all cell keys
 // when the table would be too large; inside the line the cells are found
 // by their last coordinate (cell_z), so the searches compare 32-bit values.
-__device__ __forceinline__ int32_t lower_bound_u64(const uint64_t* a, int32_t lo, int32_t hi,
-                                                   uint64_t v) {
-  while (lo < hi) {
-    const int32_t mid = (lo + hi) >> 1;
-    if (a[mid] < v)
-      lo = mid + 1;
-    else
-      hi = mid;
+// Five lower bounds lo[j] = first index in [lo[j], hi[j]) with a[i] >= v[j],
+// in lockstep (the loads of the five searches overlap).
+__device__ __forceinline__ void lower_bound5(const uint64_t* a, int32_t* lo, int32_t* hi,
+                                             const uint64_t* v) {
+  for (;;) {
+    uint64_t km[5];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      km[j] = lo[j] < hi[j] ? a[(lo[j] + hi[j]) >> 1] : 0;
+      any |= lo[j] < hi[j];
+    }
+    if (!any) return;
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+      if (lo[j] < hi[j]) {
+        const int32_t mid = (lo[j] + hi[j]) >> 1;
+        if (km[j] < v[j])
+          lo[j] = mid + 1;
+        else
+          hi[j] = mid;
+      }
   }
-  return lo;
 }
 
 __global__ __launch_bounds__(256) void k_rows(Dev d) {
@@ -708,6 +721,8 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
     const int32_t zlo = kc[last] >= 2 ? kc[last] - 2 : 0;
     const int32_t zhi = kc[last] + 2 < I.nk[last] ? kc[last] + 2 : I.nk[last] - 1;
     int32_t ls[5], le[5];
+    uint64_t k0[5];  // first key of each neighbour line (no line table)
+    bool need = false;
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
       const int r = 5 * g + j;
@@ -724,15 +739,34 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
       const int64_t line = dim == 3 ? (int64_t)r0 * I.nk[1] + r1 : (int64_t)r0;
       ls[j] = -1;
       le[j] = -1;
+      k0[j] = (uint64_t)(I.key_base + line * I.nk[last]);
       if (ok && table) {
         const int2 lr = d.lines[I.line_base + line];
         ls[j] = lr.x;
         le[j] = lr.y;
       } else if (ok) {
-        const uint64_t k0 = (uint64_t)(I.key_base + line * I.nk[last]);
-        ls[j] = lower_bound_u64(d.cell_key, 0, C, k0);
-        le[j] = lower_bound_u64(d.cell_key, ls[j], C, k0 + (uint64_t)I.nk[last]);
+        ls[j] = 0;  // searched below
+        le[j] = C;
+        need = true;
       }
+    }
+    if (need) {
+      // no line table: the line's cell range by two lower bounds over all
+      // cell keys, the five lines' searches in lockstep
+      int32_t hi[5];
+#pragma unroll
+      for (int j = 0; j < 5; ++j) hi[j] = le[j];
+      lower_bound5(d.cell_key, ls, hi, k0);
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        hi[j] = le[j];
+        le[j] = ls[j];
+        k0[j] += (uint64_t)I.nk[last];
+      }
+      lower_bound5(d.cell_key, le, hi, k0);
+#pragma unroll
+      for (int j = 0; j < 5; ++j)
+        if (le[j] <= ls[j]) ls[j] = le[j] = -1;  // empty line
     }
     // lines of more than 4 cells: first cell with z >= zlo, the five
     // searches in lockstep so that their loads overlap

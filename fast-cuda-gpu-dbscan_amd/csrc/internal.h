@@ -416,11 +416,11 @@ __device__ __forceinline__ int32_t uf_find(int32_t* par, int32_t x) {
   }
 }
 
-// Root of x by a walk that writes nothing.  The passes that flatten the
-// forest (every entry set to its root, no unions running) use it: a
-// path-halving write from another thread's walk could otherwise replace a
-// root that its owner has just stored with an intermediate ancestor, and a
-// later read of "the root" would then get an inner node.
+// Root of x by a walk that writes nothing.  The final flatten uses it (no
+// unions running): a path-halving write from another thread's walk could
+// otherwise replace a root that its owner has just stored with an inner
+// ancestor, and a later read of "the root" would then get an inner node.
+// The walk does not compress, so it is run on an already compressed forest.
 __device__ __forceinline__ int32_t uf_root(const int32_t* par, int32_t x) {
   for (;;) {
     const int32_t p = par[x];

@@ -343,13 +343,13 @@ def test_graph_replay_non_finite_input(handle, devmem, monkeypatch):
 
 @pytest.mark.parametrize("prec", PRECS)
 def test_no_line_table(handle, prec):
-    # 2-D points spread over more than 2^25 cells along x: the dense line
+    # 2-D points spread over more than 2^27 cells along x: the dense line
     # table is not built and neighbour lines are found by binary search
     rng = np.random.default_rng(21)
     n = 6000
-    centres = np.array([[0.0, 0.0], [40.0, 3.0], [1.8e7, -5.0], [1.8e7 + 30.0, 0.0]])
+    centres = np.array([[0.0, 0.0], [40.0, 3.0], [5.0e7, -5.0], [5.0e7 + 30.0, 0.0]])
     x = centres[rng.integers(0, 4, n)] + rng.normal(0, 2.0, (n, 2))
-    x[:300, 0] = rng.uniform(0, 1.8e7, 300)
+    x[:300, 0] = rng.uniform(0, 5.0e7, 300)
     st = _check(handle, x, 0.6, 4, prec)
     assert st["n_cells"] > 1000
 
