@@ -56,13 +56,14 @@ def build_dclust(verbose=False):
     for src in HOST_SOURCES:
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         objs.append(obj)
-        jobs.append(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I", INC, "-c",
+        jobs.append(["g++", "-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-I", INC, "-c",
                      os.path.join(CSRC, src), "-o", obj])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         list(ex.map(lambda c: _run(c, verbose), jobs))
     lib = os.path.join(PKG, "libdclust.so")
     tmp = lib + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp],
+         verbose)
     os.replace(tmp, lib)
     return lib
 

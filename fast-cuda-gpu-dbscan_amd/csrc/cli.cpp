@@ -5,9 +5,13 @@
 //   dclust <dataset-file> [--dim D] [--eps E] [--minpts M] [--r R]
 //          [--seeds S] [--blocks B] [--threads T] [--extra X]
 //          [--precision fp64|fp32] [--n N] [--out labels.txt] [--device I]
+//          [--write-f64 FILE]
 //
 // Prints the reference's summary (main.cu:422-432: clusters, noise, phase
-// times) and, with --out, writes one label per line (0 = noise).
+// times) and, with --out, writes one label per line (0 = noise).  The
+// dataset may be a text file or a binary dataset (dclust_write_dataset_f64);
+// --write-f64 converts the dataset to the binary format and exits without
+// clustering.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -21,7 +25,8 @@ static void usage() {
   std::fprintf(stderr,
                "usage: dclust <dataset-file> [--dim D] [--eps E] [--minpts M] [--r R]\n"
                "              [--seeds S] [--blocks B] [--threads T] [--extra X]\n"
-               "              [--precision fp64|fp32] [--n N] [--out FILE] [--device I]\n");
+               "              [--precision fp64|fp32] [--n N] [--out FILE] [--device I]\n"
+               "              [--write-f64 FILE]\n");
 }
 
 int main(int argc, char** argv) {
@@ -33,6 +38,7 @@ int main(int argc, char** argv) {
   dclust_params_default(&p);
   const char* in = nullptr;
   const char* out = nullptr;
+  const char* conv = nullptr;
   int64_t nmax = 0;
   int device = 0;
   for (int i = 1; i < argc; ++i) {
@@ -57,6 +63,7 @@ int main(int argc, char** argv) {
       p.precision = v == "fp32" ? DCLUST_PRECISION_FP32_GUARD : DCLUST_PRECISION_FP64;
     } else if (a == "--n") nmax = std::atoll(next());
     else if (a == "--out") out = next();
+    else if (a == "--write-f64") conv = next();
     else if (a == "--device") device = std::atoi(next());
     else if (a[0] == '-') {
       usage();
@@ -81,6 +88,15 @@ int main(int argc, char** argv) {
     return 1;
   }
   std::printf("Preprocessed %lld data in dataset\n", (long long)n);
+  if (conv) {
+    rc = dclust_write_dataset_f64(conv, xyz.data(), n, p.dim);
+    if (rc) {
+      std::fprintf(stderr, "Error writing %s: %s\n", conv, dclust_strerror(rc));
+      return 1;
+    }
+    std::printf("Wrote binary dataset %s\n", conv);
+    return 0;
+  }
   dclust_handle* h = nullptr;
   rc = dclust_create(&h, device);
   if (rc) {

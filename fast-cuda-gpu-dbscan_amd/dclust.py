@@ -107,6 +107,7 @@ def _lib():
         L.dclust_import_dataset.argtypes = [C.c_char_p, C.c_int32, C.c_int64, _P,
                                             C.POINTER(C.c_int64)]
         L.dclust_write_labels.argtypes = [C.c_char_p, _P, C.c_int64]
+        L.dclust_write_dataset_f64.argtypes = [C.c_char_p, _P, C.c_int64, C.c_int32]
         L.dclust_strerror.argtypes = [C.c_int]
         L.dclust_strerror.restype = C.c_char_p
         L.dclust_version.restype = C.c_char_p
@@ -277,6 +278,16 @@ def import_dataset(path: str, dim: int, max_points: int = 0):
     _check(_lib().dclust_import_dataset(path.encode(), dim, max_points, out.ctypes.data,
                                         C.byref(n)), "dclust_import_dataset")
     return out
+
+
+def write_dataset_f64(path: str, xyz) -> None:
+    """Binary dataset file (24-byte header, row-major doubles), read back by
+    import_dataset."""
+    x = np.ascontiguousarray(xyz, dtype=np.float64)
+    if x.ndim != 2:
+        raise ValueError("xyz must be (n, dim)")
+    _check(_lib().dclust_write_dataset_f64(path.encode(), x.ctypes.data, x.shape[0], x.shape[1]),
+           "dclust_write_dataset_f64")
 
 
 def write_labels(path: str, labels) -> None:

@@ -207,6 +207,16 @@ int dclust_autotune_device(dclust_handle* h, int32_t count, const double* d_xyz,
 int dclust_import_dataset(const char* path, int32_t dim, int64_t max_points,
                           double* out, int64_t* n_points);
 
+/* The dataset reader above also reads binary datasets (no counterpart in the
+ * reference): a 24-byte header -- the 8 bytes "DCLUSTF8", the point count
+ * (int64), the dimension (int32), 4 zero bytes -- then the points as
+ * row-major little-endian doubles.  A file that starts with the header is
+ * read as binary; its dimension must equal `dim` (else DCLUST_E_PARAM), its
+ * size must match the count (else DCLUST_E_IO).  Text files are parsed by
+ * up to 16 threads with the sequential result.  This writes the binary
+ * format. */
+int dclust_write_dataset_f64(const char* path, const double* xyz, int64_t n, int32_t dim);
+
 /* Writes one label per line (0 = noise), the file format read by
  * test-results/cluster-check.cpp:201-224. */
 int dclust_write_labels(const char* path, const int32_t* labels, int64_t n);
