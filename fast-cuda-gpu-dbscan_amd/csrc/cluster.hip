@@ -1039,6 +1039,8 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
   int left = budget;
   int sl = 0;  // seed-list length (wave-uniform)
   long long pops = 0, ovf = 0, edges = 0, seeds = 0, rounds = 0;
+  int32_t cur = -1;    // chain of the cells in the list when not mixed
+  bool mixed = false;  // the list holds queue cells (chains read per cell)
   for (;;) {
     if (left <= 0) {
       if (sl > 0) {
@@ -1061,6 +1063,7 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
           if (lane < cnt) list[lane] = qin[base + lane];
           sl = cnt;
           got = true;
+          mixed = true;  // queue cells: chains of other waves
         } else {
           qlive = false;
         }
@@ -1084,6 +1087,8 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
             ++seeds;
             sl = 1;
             got = true;
+            cur = (int32_t)(seg + l);
+            mixed = false;
             seg = seg + l + 1;
             break;
           }
@@ -1106,9 +1111,14 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
     int32_t own = 0;
     unsigned long long m0 = 0;
     if (act) {
-      // coherent read: the claim may have been made moments ago on another
-      // XCD, whose L2 is not coherent with this one for plain loads
-      own = __hip_atomic_load(&d.cell_label[A], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // A list filled only by this wave's seeding and pushes holds cells of
+      // its current chain alone (a new chain is seeded only into an empty
+      // list).  After a refill from the queue the chain is read: coherently,
+      // since the claim may have been made moments ago on another XCD, whose
+      // L2 is not coherent with this one for plain loads.
+      own = mixed ? __hip_atomic_load(&d.cell_label[A], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT)
+                  : cur;
       m0 = d.amask[A];
     }
     // the popped cells' chains, masks and near lines go to LDS (the lines
