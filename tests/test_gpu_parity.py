@@ -398,3 +398,34 @@ def test_back_to_back_runs_stable(devmem):
                 assert not diff, f"sweep {sweep}: instances {diff} changed"
     finally:
         h.close()
+
+
+def _degenerate_cases():
+    rng = np.random.default_rng(77)
+    line = np.zeros((3000, 3))
+    line[:, 0] = rng.uniform(0, 1, 3000)  # zero extent on two axes
+    cases = {
+        "single point": (np.array([[0.25, -3.0, 7.0]]), 0.1, 1),
+        "single point, minpts 2": (np.array([[0.25, -3.0]]), 0.1, 2),
+        "all identical": (np.full((5000, 3), 1.5), 0.01, 10),
+        "identical, minpts above n": (np.full((40, 2), -2.0), 0.01, 41),
+        "minpts 1 (every point core)": (rng.uniform(0, 1, (3000, 2)), 0.005, 1),
+        "minpts above n": (rng.uniform(0, 1, (500, 3)), 0.2, 600),
+        "collinear in 3-D": (line, 0.002, 4),
+        "negative offset coordinates": (rng.normal(-1e4, 1.0, (6000, 2)), 0.05, 5),
+        "eps covers everything": (rng.uniform(0, 1, (2000, 2)), 10.0, 3),
+        "duplicates within clusters": (np.repeat(rng.uniform(0, 1, (800, 3)), 4, axis=0), 0.03, 6),
+    }
+    return cases
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("case", sorted(_degenerate_cases()))
+def test_degenerate_inputs(handle, prec, case):
+    x, eps, mp = _degenerate_cases()[case]
+    _check(handle, x, eps, mp, prec)
+
+
+def test_empty_input(handle):
+    lab = handle.run(np.empty((0, 3)), 0.1, 4, precision=dclust.FP32_GUARD)
+    assert lab.shape == (0,) and handle.last_stats["n_points"] == 0
