@@ -60,6 +60,7 @@ struct dclust_handle {
   std::vector<uint8_t> prev_sig;   // launch state of the last eager run
   int prev_npass = 0;              // its expansion passes
   long long prev_nlines = 0;       // its neighbour-line count
+  int prev_sort_passes = 0;        // its radix passes
   hipGraphExec_t gexec = nullptr;  // captured run
   std::vector<uint8_t> gsig;       // launch state it was captured for
   int gpasses = 0;                 // expansion passes it holds
@@ -272,11 +273,13 @@ hipError_t enqueue_tail(dclust_handle* h, const Dev& d, bool f32, int grid_cells
   return hipGetLastError();
 }
 
-// A whole run with no host read in between, captured into a graph: every
-// radix pass is launched (passes beyond the key width exit on the device)
-// and `passes` expansion passes (a pass with nothing pending exits at once).
+// A whole run with no host read in between, captured into a graph:
+// `sort_passes` radix passes (the key width of the previous run; a wider key
+// abandons the run on the device) and `passes` expansion passes (a pass with
+// nothing pending exits at once).
 hipError_t capture_run(dclust_handle* h, const Dev& d, const RunShape& shape, bool f32,
-                       int nchunks, int grid_cells, int passes, hipGraphExec_t* out) {
+                       int nchunks, int grid_cells, int sort_passes, int passes,
+                       hipGraphExec_t* out) {
   const hipStream_t s = h->stream;
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
@@ -284,7 +287,7 @@ hipError_t capture_run(dclust_handle* h, const Dev& d, const RunShape& shape, bo
   hipError_t body = mark(h, E_START, true);
   if (body == hipSuccess) {
     enqueue_head(d, nchunks, s);
-    launch_index_tail(d, d.n, nchunks, grid_cells, kMaxPasses, s);
+    launch_index_tail(d, d.n, nchunks, grid_cells, sort_passes, s);
     body = enqueue_core(h, d, shape, f32, true);
   }
   for (int p = 0; body == hipSuccess && p < passes; ++p) {
@@ -545,7 +548,8 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
       dg.line_cap = cap;
       const int gp = std::max(2, h->prev_npass + 1);
       hipGraphExec_t ge = nullptr;
-      if (capture_run(h, dg, shape, f32, nchunks, grid_cells, gp, &ge) == hipSuccess) {
+      const int sp = std::max(1, std::min(kMaxPasses, h->prev_sort_passes));
+      if (capture_run(h, dg, shape, f32, nchunks, grid_cells, sp, gp, &ge) == hipSuccess) {
         h->gexec = ge;
         h->gsig = sig;
         h->gpasses = gp;
@@ -627,6 +631,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
       h->prev_sig.swap(sig);
       h->prev_npass = npass;
       h->prev_nlines = nlines;
+      h->prev_sort_passes = passes;
     }
     return rc;
   }

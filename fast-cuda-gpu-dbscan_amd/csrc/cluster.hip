@@ -374,6 +374,7 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
 
 template <bool F32>
 __global__ __launch_bounds__(256) void k_core(Dev d) {
+  if (abandoned(d)) return;
   const int64_t pl = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const bool act = pl < d.n;
   const int32_t p = act ? (int32_t)pl : 0;
@@ -653,6 +654,7 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
 
 template <bool F32>
 __global__ __launch_bounds__(256) void k_adj(Dev d) {
+  if (abandoned(d)) return;
   __shared__ unsigned long long s_m[4][64];
   __shared__ long long s_pre[kStripes + 1];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -819,6 +821,7 @@ __device__ void adj_heavy(const Dev& d, const Inst& I, const Par& P, int32_t A, 
 
 template <bool F32>
 __global__ __launch_bounds__(64) void k_adj_heavy(Dev d) {
+  if (abandoned(d)) return;
   __shared__ int32_t s_qi[kQCache];
   __shared__ uint32_t s_qg[kQCache * 3];
   __shared__ double s_qx[F32 ? 1 : kQCache * 3];
@@ -1010,7 +1013,7 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
   __shared__ unsigned long long s_seen[4][kSeen];
   __shared__ uint8_t s_tag[4][kSeen];  // per-wave dedup slots of one step
   __shared__ PopStage s_pop[4];         // per-wave popped cells of a round
-  if (d.ctl[C_IDLE]) return;
+  if (d.ctl[C_IDLE] || abandoned(d)) return;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   const int nwb = blockDim.x >> 6;
   const int64_t gw = (int64_t)xcd_block(blockIdx.x, gridDim.x) * nwb + wv;
@@ -1259,7 +1262,7 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
 // (queue or seeding segments) marks the run as abandoned, and the kernels
 // after the expansion return at once (the host runs the call again).
 __global__ void k_expand_check(Dev d, int passes) {
-  d.ctl[C_ABANDON] = d.ctl[C_Q0_N + (passes & 1)] != 0 || d.ctl[C_SEGS_LEFT] != 0;
+  if (d.ctl[C_Q0_N + (passes & 1)] != 0 || d.ctl[C_SEGS_LEFT] != 0) d.ctl[C_ABANDON] = 1;
 }
 
 // Resets the counters of the queue that pass `pass` fills and of the head
@@ -1275,6 +1278,7 @@ __global__ void k_pass_prep(Dev d, int pass) {
 // After a pass: queue entries no wave took (every wave spent its budget)
 // are appended to the queue of the next pass.  One block.
 __global__ __launch_bounds__(256) void k_pass_carry(Dev d, int pass) {
+  if (abandoned(d)) return;
   if (pass == 0) return;
   const long long qn = d.ctl[C_Q0_N + (pass & 1)];
   const long long head = d.ctl[C_Q0_HEAD + (pass & 1)];

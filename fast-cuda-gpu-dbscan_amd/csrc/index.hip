@@ -506,6 +506,7 @@ __device__ __forceinline__ const uint64_t* sorted_keys(const Dev& d) {
 }
 
 __global__ __launch_bounds__(CS_T) void k_csr_count(Dev d) {
+  if (abandoned(d)) return;
   const uint64_t* key = sorted_keys(d);
   const int64_t t0 = (int64_t)blockIdx.x * CS_TILE;
   int c = 0;
@@ -583,6 +584,7 @@ __device__ __forceinline__ void unpack_cell(int dim, uint64_t pc, int32_t* kc) {
 }
 
 __global__ __launch_bounds__(CS_T) void k_csr_write(Dev d) {
+  if (abandoned(d)) return;
   const uint64_t* key = sorted_keys(d);
   const int64_t t0 = (int64_t)blockIdx.x * CS_TILE;
   __shared__ int s_w[4];
@@ -625,6 +627,7 @@ __global__ __launch_bounds__(CS_T) void k_csr_write(Dev d) {
 
 // ---------------------------------------------------------------- gather --
 __global__ __launch_bounds__(256) void k_gather(Dev d) {
+  if (abandoned(d)) return;
   const int32_t* vsrc = d.val[sort_parity(d, kMaxPasses)];
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < d.n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -655,6 +658,7 @@ __device__ __forceinline__ int64_t cell_line(const Dev& d, int32_t c) {
 // Line table: for every possible line, the range of its non-empty cells in
 // the sorted cell array (-1 when empty).
 __global__ __launch_bounds__(256) void k_line_clear(Dev d) {
+  if (abandoned(d)) return;
   const long long P = d.ctl[C_NLINES];
   if (P > d.line_cap) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P;
@@ -663,6 +667,7 @@ __global__ __launch_bounds__(256) void k_line_clear(Dev d) {
 }
 
 __global__ __launch_bounds__(256) void k_line_mark(Dev d) {
+  if (abandoned(d)) return;
   const int32_t C = (int32_t)d.ctl[C_NCELL];
   if (d.ctl[C_NLINES] > d.line_cap) return;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < C;
@@ -705,6 +710,7 @@ __device__ __forceinline__ void lower_bound5(const uint64_t* a, int32_t* lo, int
 }
 
 __global__ __launch_bounds__(256) void k_rows(Dev d) {
+  if (abandoned(d)) return;
   const int32_t C = (int32_t)d.ctl[C_NCELL];
   const bool table = d.ctl[C_NLINES] <= d.line_cap;
   const int g = blockIdx.y;  // lines 5g .. 5g+4 of the stencil
@@ -833,6 +839,13 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
   stat_add(d, C_NBRC, nbrc);
 }
 
+// After a fixed number of radix passes: a key wider than they cover leaves
+// the keys unsorted, so the run is abandoned (every later kernel returns at
+// once and the host runs the call again).
+__global__ void k_sort_check(Dev d, int passes) {
+  if (d.ctl[C_KEYBITS] > 8 * passes) d.ctl[C_ABANDON] = 1;
+}
+
 // ------------------------------------------------------------- launchers --
 void launch_index_head(const Dev& d, int nchunks, hipStream_t s) {
   hipLaunchKernelGGL(k_bbox, dim3(nchunks), dim3(256), 0, s, d);
@@ -850,6 +863,7 @@ void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
     hipLaunchKernelGGL(k_rs_rowscan, dim3(256), dim3(256), 0, s, d, p, nblk);
     hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
   }
+  if (passes < kMaxPasses) hipLaunchKernelGGL(k_sort_check, dim3(1), dim3(1), 0, s, d, passes);
   const int ncs = (int)((n + CS_TILE - 1) / CS_TILE);
   hipLaunchKernelGGL(k_csr_count, dim3(ncs), dim3(CS_T), 0, s, d);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, d.scan_tmp, ncs, (long long*)nullptr);

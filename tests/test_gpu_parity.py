@@ -429,3 +429,22 @@ def test_degenerate_inputs(handle, prec, case):
 def test_empty_input(handle):
     lab = handle.run(np.empty((0, 3)), 0.1, 4, precision=dclust.FP32_GUARD)
     assert lab.shape == (0,) and handle.last_stats["n_points"] == 0
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_graph_replay_reruns_when_the_key_widens(handle, prec, monkeypatch):
+    # a graph holds the radix passes of the key width it was captured after;
+    # coordinates whose cell key needs more bits abandon the replay on the
+    # device and the call runs eagerly; the next identical call recaptures
+    monkeypatch.delenv("DCLUST_GRAPH", raising=False)
+    rng = np.random.default_rng(44)
+    n, eps, mp = 5000, 0.01, 4
+    small = rng.uniform(0, 0.05, (n, 2))  # ~100 cells: one 8-bit pass
+    big = rng.uniform(0, 10.0, (n, 2))    # ~4M cells: three passes
+    _check(handle, small[: n // 2], eps, mp, prec)
+    out = []
+    for x in [small, small, big, big, small]:
+        st = _check(handle, x, eps, mp, prec)
+        out.append((st["diag"][14], st["key_bits"]))
+    assert [r for r, _ in out] == [0, 1, 0, 1, 1]
+    assert out[1][1] <= 8 and out[2][1] > 16
