@@ -555,20 +555,12 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
 #pragma unroll
   for (int k = 0; k < D; ++k) fr[k] = (int32_t)(pg[k] & kFrac);
   const int32_t zA = act ? d.cell_z[c] : 0;
-  // the near lines of the cell, loaded together
-  int32_t rn[NN];
-  {
-    const RowsOf rows = rows_of(d, c);
-#pragma unroll
-    for (int r = 0; r < NN; ++r) rn[r] = act ? rows[r] : 0;
-  }
+  const RowsOf rows = rows_of(d, c);
   for (int step = 0; step < NN; ++step) {
     const int r = (step + rank) % NN;
     const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
     const int o1 = D == 3 ? kRowOff3[r][1] : 0;
-    int32_t pk = rn[0];
-#pragma unroll
-    for (int i = 1; i < NN; ++i) pk = r == i ? rn[i] : pk;
+    const int32_t pk = act ? rows[r] : 0;
     const int32_t len = pk >> kRowCntShift;
     if (len > 0) {
       const int32_t st = pk & kRowCellMask;
@@ -653,7 +645,9 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
 }
 
 template <bool F32>
-__global__ __launch_bounds__(256) void k_adj(Dev d) {
+// 5 waves per SIMD (96 VGPRs, a few bytes of spill): the lanes are latency
+// bound, so residency pays more than the registers
+__global__ __launch_bounds__(256, 5) void k_adj(Dev d) {
   if (abandoned(d)) return;
   __shared__ unsigned long long s_m[4][64];
   __shared__ long long s_pre[kStripes + 1];
