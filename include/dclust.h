@@ -149,10 +149,10 @@ int dclust_run(dclust_handle* h, const double* xyz, int64_t n,
  * switches all equal the previous call's is captured once into a hipGraph
  * and replayed from then on (the coordinates may change between calls).
  * A replay launches the whole run at once with no host read in between:
- * every radix pass (passes beyond the key width exit on the device) and a
- * fixed number of expansion passes, one more than the previous run needed.
- * When the replay finds that the expansion needed more, the call reruns
- * eagerly and the next identical call captures a longer graph.  The
+ * the radix passes of the previous run's key width and a fixed number of
+ * expansion passes, one more than the previous run needed.  When the
+ * replay finds that the key or the expansion needed more, the call reruns
+ * eagerly and the next identical call captures a new graph.  The
  * environment variable DCLUST_GRAPH=0 turns replay off.  This applies to
  * every entry point that runs a clustering (dclust_run, the batched ones). */
 int dclust_run_device(dclust_handle* h, const double* d_xyz, int64_t n,
