@@ -686,8 +686,8 @@ __global__ __launch_bounds__(256) void k_line_mark(Dev d) {
 // by their last coordinate (cell_z), so the searches compare 32-bit values.
 // Five lower bounds lo[j] = first index in [lo[j], hi[j]) with a[i] >= v[j],
 // in lockstep (the loads of the five searches overlap).
-__device__ __forceinline__ void lower_bound5(const uint64_t* a, int32_t* lo, int32_t* hi,
-                                             const uint64_t* v) {
+__device__ __forceinline__ void lower_bound5(const uint64_t* a, int32_t (&lo)[5],
+                                             int32_t (&hi)[5], const uint64_t (&v)[5]) {
   for (;;) {
     uint64_t km[5];
     bool any = false;
@@ -697,7 +697,6 @@ __device__ __forceinline__ void lower_bound5(const uint64_t* a, int32_t* lo, int
       any |= lo[j] < hi[j];
     }
     if (!any) return;
-#pragma unroll
     for (int j = 0; j < 5; ++j)
       if (lo[j] < hi[j]) {
         const int32_t mid = (lo[j] + hi[j]) >> 1;
