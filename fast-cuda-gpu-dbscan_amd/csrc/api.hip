@@ -221,10 +221,7 @@ hipError_t mark(dclust_handle* h, int e, bool graph) {
 
 // counters, bounding boxes, grid set-up
 void enqueue_head(const Dev& d, int nchunks, hipStream_t s) {
-  launch_fill_i64(d.ctl, C_NCTL, 0, s);
-  launch_fill_i64(d.stat, (int64_t)C_NCTL * kStripes, 0, s);
-  launch_fill_i32(d.pass_scatter, kMaxPasses, 0, s);
-  launch_bbox_init(d, s);
+  launch_init_head(d, s);
   launch_index_head(d, nchunks, s);
 }
 
@@ -232,15 +229,9 @@ void enqueue_head(const Dev& d, int nchunks, hipStream_t s) {
 hipError_t enqueue_core(dclust_handle* h, const Dev& d, const RunShape& shape, bool f32,
                         bool graph) {
   const hipStream_t s = h->stream;
-  const int64_t n = d.n;
   hipError_t e;
   if ((e = mark(h, E_INDEX, graph)) != hipSuccess) return e;
-  launch_fill_i32(d.cell_label, n, kUnclaimed, s);
-  launch_iota(d.parent, n, s);
-  launch_fill_i32(d.compmin, n, kNoKey, s);
-  launch_fill_u8(d.flag, n, 0, s);
-  launch_fill_i32(d.pair_cnt, shape.waves, 0, s);
-  launch_fill_i32(d.cell_ncore, n, 0, s);
+  launch_init_run(d, shape.waves, s);
   // the core (neighbour-count) kernel is bracketed by its own events
   if ((e = mark(h, E_CORE0, graph)) != hipSuccess) return e;
   launch_core(d, f32, s);

@@ -1610,7 +1610,33 @@ __global__ __launch_bounds__(256) void k_output(Dev d) {
   stat_add(d, C_NNOISE, noise);
 }
 
+// Per-run state of the clustering phases, one launch: chain labels of the
+// cells (unclaimed), the chain forest (every id its own root), component
+// minima, component flags, per-cell core counts (all over the n entries
+// that bound the cells and chain ids) and the per-wave pair counts.
+__global__ void k_init_run(Dev d, int64_t waves) {
+  const int64_t m = d.n > waves ? d.n : waves;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < d.n) {
+      d.cell_label[i] = kUnclaimed;
+      d.parent[i] = (int32_t)i;
+      d.compmin[i] = kNoKey;
+      d.flag[i] = 0;
+      d.cell_ncore[i] = 0;
+    }
+    if (i < waves) d.pair_cnt[i] = 0;
+  }
+}
+
 // ------------------------------------------------------------- launchers --
+void launch_init_run(const Dev& d, int64_t waves, hipStream_t s) {
+  const int64_t m = d.n > waves ? d.n : waves;
+  int64_t b = (m + 255) / 256;
+  b = b > 4096 ? 4096 : (b < 1 ? 1 : b);
+  hipLaunchKernelGGL(k_init_run, dim3((int)b), dim3(256), 0, s, d, waves);
+}
+
 // one lane per point, grid a multiple of 8 (XCD remap)
 static int point_grid(int64_t n) {
   int64_t b = (n + 255) / 256;

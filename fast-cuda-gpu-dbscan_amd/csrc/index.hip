@@ -23,19 +23,6 @@
 namespace dclust {
 
 // ------------------------------------------------------------------ fill --
-template <class T>
-__global__ void k_fill(T* p, int64_t n, T v) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = v;
-}
-
-__global__ void k_iota(int32_t* p, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = (int32_t)i;
-}
-
 static int fill_grid(int64_t n) {
   int64_t b = (n + 255) / 256;
   if (b > 4096) b = 4096;
@@ -43,36 +30,34 @@ static int fill_grid(int64_t n) {
   return (int)b;
 }
 
-void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_fill<int32_t>, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v);
+// Start of a run, one launch: control counters, statistics stripes, radix
+// pass flags and the bounding-box stripes (per instance and stripe: 0..2 min
+// as ordered keys starting at the largest key, 3..5 max, 6 max |x|, 7
+// non-finite flag, all starting at 0).
+__global__ void k_init_head(Dev d) {
+  const int64_t nb = (int64_t)d.ninst * kBoxStripes * 8;
+  const int64_t ns = (int64_t)C_NCTL * kStripes;
+  const int64_t tot = nb + ns + C_NCTL + kMaxPasses;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < nb)
+      d.bbox[i] = (i & 7) < 3 ? ~0ull : 0ull;
+    else if (i < nb + ns)
+      d.stat[i - nb] = 0;
+    else if (i < nb + ns + C_NCTL)
+      d.ctl[i - nb - ns] = 0;
+    else
+      d.pass_scatter[i - nb - ns - C_NCTL] = 0;
+  }
 }
-void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_fill<uint8_t>, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v);
-}
-void launch_fill_i64(long long* p, int64_t n, long long v, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_fill<long long>, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v);
-}
-void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(fill_grid(n)), dim3(256), 0, s, p, n, v);
-}
-void launch_iota(int32_t* p, int64_t n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_iota, dim3(fill_grid(n)), dim3(256), 0, s, p, n);
+
+void launch_init_head(const Dev& d, hipStream_t s) {
+  const int64_t tot = (int64_t)d.ninst * kBoxStripes * 8 + (int64_t)C_NCTL * kStripes + C_NCTL +
+                      kMaxPasses;
+  hipLaunchKernelGGL(k_init_head, dim3(fill_grid(tot)), dim3(256), 0, s, d);
 }
 
 // ------------------------------------------------------------------ bbox --
-// slots per instance and stripe: 0..2 min (ordered keys, start at the
-// largest key), 3..5 max, 6 max |x|, 7 non-finite flag (all start at 0)
-__global__ void k_bbox_init(Dev d) {
-  const int64_t n = (int64_t)d.ninst * kBoxStripes * 8;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    d.bbox[i] = (i & 7) < 3 ? ~0ull : 0ull;
-}
-
-void launch_bbox_init(const Dev& d, hipStream_t s) {
-  hipLaunchKernelGGL(k_bbox_init, dim3(fill_grid((int64_t)d.ninst * kBoxStripes * 8)),
-                     dim3(256), 0, s, d);
-}
 
 // One block per chunk (chunks never straddle instances).  Block reduction,
 // then one ordered-u64 atomic min/max per axis.  Non-finite input sets the

@@ -4,13 +4,11 @@
 
 namespace dclust {
 
-void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
-void launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
-void launch_fill_i64(long long* p, int64_t n, long long v, hipStream_t s);
-void launch_fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
-void launch_iota(int32_t* p, int64_t n, hipStream_t s);
-
-void launch_bbox_init(const Dev& d, hipStream_t s);
+// start of a run: control counters, statistics, radix pass flags, bbox
+void launch_init_head(const Dev& d, hipStream_t s);
+// per-run state of the clustering phases (labels, forest, minima, flags,
+// per-cell core counts, per-wave pair counts)
+void launch_init_run(const Dev& d, int64_t waves, hipStream_t s);
 // bbox + grid set-up (per-instance cell width, cells per axis, key space)
 void launch_index_head(const Dev& d, int nchunks, hipStream_t s);
 // cell keys, `passes` radix passes, CSR compaction, gather, neighbour rows
