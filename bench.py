@@ -143,6 +143,49 @@ def cpu_baseline(x, info, budget_s=15.0):
                        f"{dt:.2f} s")}
 
 
+def host_cpu():
+    """Model name and the cores this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "cpus_usable": len(os.sched_getaffinity(0)),
+            "cpus_online": os.cpu_count()}
+
+
+def cpu_baseline_multicore(x, info, budget_s=10.0):
+    """multicore-cpu/dbscan.cpp restated (oracle_multicore_dbscan): FP32, one
+    OpenMP scan over all points per expanded point, so O(N^2); timed on a
+    sub-box sample with up to 16 threads (dbscan.cpp:115-117 uses 16)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # CPU baseline leg only
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    eps, mp = info["eps"], info["minpts"]
+    m = 20000
+    while True:
+        xs, a = spatial_sample(x, min(m, x.shape[0]))
+        t0 = time.perf_counter()
+        k, _ = oracle.multicore_dbscan(xs, eps, mp - 1, nthreads=threads)
+        dt = time.perf_counter() - t0
+        if dt >= budget_s / 4 or xs.shape[0] >= x.shape[0]:
+            break
+        m = int(m * min(8.0, max(1.5, (budget_s / 2 / max(dt, 1e-3)) ** 0.5)))
+    n = x.shape[0]
+    return {"value": round(xs.shape[0] / dt / 1e6, 6), "unit": "Mpoints/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"multicore-cpu/dbscan.cpp restated (oracle/oracle.c "
+                       f"oracle_multicore_dbscan: FP32, sqrt compare, minpts {mp - 1} others), "
+                       f"{xs.shape[0]} points of the same input inside the sub-box "
+                       f"[0,{a:.4f}]^{x.shape[1]}, {k} clusters reported, {dt:.2f} s"),
+            "full_input_estimate_s": round(dt * (n / xs.shape[0]) ** 2, 1)}
+
+
 BATCH_DESC = ("cfg4: 4,096 independent 2-D/3-D instances, sizes log-uniform 200-200,000, "
               "2-32 Gaussian blobs + 5% noise each, eps 0.5-2x the median 10-NN distance, "
               "minpts in {4,8,16}, FP32 + guard band")
@@ -316,7 +359,11 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(args.workload, args.precision),
-                     "algorithmic_bytes": int(bytes_core), "avg_launch_ms": round(t_core, 5)},
+                     "algorithmic_bytes": int(bytes_core), "avg_launch_ms": round(t_core, 5),
+                     # pairs a scan of every point's 5^D half-eps stencil would test, and the
+                     # distance evaluations the kernel makes, per second of its launch
+                     "stencil_pairs_per_s": round(st["cand_total"] / (t_core * 1e-3), 1),
+                     "distance_evals_per_s": round(st["pairs_evaluated"] / (t_core * 1e-3), 1)},
         "phases_ms": phases,
         "counters": {k: st[k] for k in ("n_cells", "n_core", "n_noise", "n_clusters", "n_chains",
                                         "n_collisions", "n_overflow", "n_passes",
@@ -369,6 +416,9 @@ def main():
     if not args.no_cpu_baseline:
         log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(x, info)
+        out["cpu_baseline"]["host"] = host_cpu()
+        log("CPU baseline, multicore restatement")
+        out["cpu_baseline_multicore"] = cpu_baseline_multicore(x, info)
     h.close()
     hip.free(d_x)
     hip.free(d_lab)

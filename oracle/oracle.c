@@ -498,7 +498,8 @@ int oracle_multicore_dbscan(const double* xyz, int64_t n, int dim, double eps,
   free(cl);
   free(row);
   free(queue);
-  return 0;
+  /* the program's one result: clusters started (dbscan.cpp:84-95) */
+  return (int)(next_id - 1);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -47,8 +47,11 @@ int oracle_brute_counts(const double* xyz, int64_t n, int dim, double eps,
 
 /* Restatement of multicore-cpu/dbscan.cpp:56-209 (FP32, sqrt compare, self
  * excluded from the count, BFS expansion, O(N) OpenMP scan per expanded
- * point).  Timing baseline only: its labels follow that program's own
- * quirks.  labels: >0 cluster, 0 otherwise. */
+ * point).  Timing baseline only.  Returns the number of clusters that
+ * program reports (dbscan.cpp:84-101), or a negative value when out of
+ * memory.  Its per-point labels are not a clustering: a core point's own row
+ * includes itself, so the pass that labels it marks it again (MARKED = 0,
+ * dbscan.cpp:187-199), and labels[] (>0 cluster, 0 otherwise) ends all 0. */
 int oracle_multicore_dbscan(const double* xyz, int64_t n, int dim, double eps,
                             int minpts, int nthreads, int32_t* labels);
 

@@ -97,11 +97,15 @@ def brute_counts(xyz, eps):
 
 
 def multicore_dbscan(xyz, eps, minpts, nthreads=0):
+    """(number of clusters reported, labels) of the multicore-cpu restatement;
+    minpts counts the other points only, as that program does."""
     x = _a(xyz)
     lab = np.empty(x.shape[0], dtype=np.int32)
-    lib().oracle_multicore_dbscan(x.ctypes.data, x.shape[0], x.shape[1], eps, minpts, nthreads,
-                                  lab.ctypes.data)
-    return lab
+    rc = lib().oracle_multicore_dbscan(x.ctypes.data, x.shape[0], x.shape[1], eps, minpts,
+                                       nthreads, lab.ctypes.data)
+    if rc < 0:
+        raise MemoryError("oracle_multicore_dbscan")
+    return rc, lab
 
 
 def cluster_check(c1, c2):

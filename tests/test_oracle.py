@@ -8,7 +8,10 @@
 - the predicate against hand-built boundary pairs (closed box, exact eps^2
   ties, FMA-sensitive sums) whose answers are fixed by IEEE arithmetic;
 - the cluster-check restatement against the reference checker compiled in
-  place, and against hand-computed Jaccard scores.
+  place, and against hand-computed Jaccard scores;
+- the multicore-cpu restatement (a timing baseline) by its reported cluster
+  count: the minpts threshold without the point itself, the <= tie, expansion
+  through cores, and the exact oracle's count where no pair lies near eps.
 """
 import hashlib
 import os
@@ -183,3 +186,68 @@ def test_cluster_check_hand_values():
     ne, eq, lost, score = oracle.cluster_check([1, 1, 2, 2], [1, 1, 1, 0])
     assert ne == 1 and eq == 0
     assert score == pytest.approx((2 / 3 + 2 / 3 + 1 / 4 + 0) / 4)
+
+
+# multicore-cpu/dbscan.cpp restatement (the timed 16-thread baseline).  That
+# program's one result is the number of clusters it reports; its minpts counts
+# the other points only (dbscan.cpp:143) and it compares the FP32 sqrt distance
+# with <= (dbscan.cpp:131-136).  Where no pair lies near eps, its clusters are
+# the components of the core graph, i.e. the exact oracle's clusters at
+# minpts + 1.
+
+def _groups(rng, sizes, dim, eps):
+    """Groups of points inside cubes of side eps/4 (every pair within eps), the
+    cubes 10 eps apart, shuffled."""
+    pts = []
+    for g, m in enumerate(sizes):
+        c = np.zeros(dim)
+        c[0] = 10 * eps * g
+        pts.append(c + rng.uniform(0, eps / 4, size=(m, dim)))
+    x = np.concatenate(pts)
+    return x[rng.permutation(x.shape[0])]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_multicore_minpts_excludes_self(dim):
+    rng = np.random.default_rng(11)
+    eps, mp = 0.1, 4
+    # groups of 4 points: 3 others each -> noise; groups of >= 5 -> one cluster each
+    x = _groups(rng, [4, 5, 5, 1, 7, 4, 12], dim, eps)
+    k, lab = oracle.multicore_dbscan(x, eps, mp, nthreads=2)
+    assert k == 4
+    assert int(oracle.seq_dbscan(x, eps, mp + 1).max()) == 4
+    # every labelled point is marked again by its own row (dbscan.cpp:187-199)
+    assert not lab.any()
+
+
+def test_multicore_distance_tie_is_inside():
+    # sqrt(0.25) == 0.5 exactly: the pair is within eps, each has 1 other point
+    assert oracle.multicore_dbscan(np.array([[0.0, 0.0], [0.5, 0.0]]), 0.5, 1)[0] == 1
+    assert oracle.multicore_dbscan(np.array([[0.0, 0.0], [0.5000001, 0.0]]), 0.5, 1)[0] == 0
+
+
+def test_multicore_expands_through_cores():
+    # two chains of 10 points 0.6 eps apart, interior points have 2 others:
+    # each chain is one cluster, reached point by point from its first core
+    eps = 1.0
+    t = np.arange(10) * 0.6
+    a = np.stack([t, np.zeros(10)], axis=1)
+    b = np.stack([t + 100.0, np.full(10, 50.0)], axis=1)
+    x = np.concatenate([a, b])[np.random.default_rng(3).permutation(20)]
+    assert oracle.multicore_dbscan(x, eps, 2)[0] == 2
+    assert oracle.multicore_dbscan(x, eps, 3)[0] == 0
+
+
+@pytest.mark.parametrize("dim,seed", [(2, 0), (3, 6)])
+def test_multicore_cluster_count_matches_oracle(dim, seed):
+    rng = np.random.default_rng(seed)
+    cent = rng.uniform(0, 1, size=(12, dim))
+    x = np.concatenate([c + rng.normal(0, 0.02, size=(250, dim)) for c in cent]
+                       + [rng.uniform(0, 1, size=(300, dim))])
+    eps, mp = 0.012 if dim == 2 else 0.025, 6
+    # precondition: no pair within 1e-5 relative of eps, so FP32 and FP64 agree
+    d = np.sqrt(((x[:, None, :] - x[None, :, :]) ** 2).sum(-1))
+    assert not (np.abs(d - eps) < 1e-5 * eps).any()
+    k, _ = oracle.multicore_dbscan(x, eps, mp - 1, nthreads=2)
+    ref = oracle.seq_dbscan(x, eps, mp)
+    assert ref.max() > 3 and k == int(ref.max())
