@@ -186,6 +186,60 @@ def cpu_baseline_multicore(x, info, budget_s=10.0):
             "full_input_estimate_s": round(dt * (n / xs.shape[0]) ** 2, 1)}
 
 
+def onestep_leg(hip, h, x, info, prec, budget_s=8.0):
+    """The one-step baseline (CUDA-DClust design, dclust_run_onestep_device)
+    and the main path on the same sub-box sample of the workload, grown until
+    the baseline takes about budget_s / 4."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # checker only
+
+    n, dim = x.shape
+    m = 20000
+    while True:
+        xs, a = spatial_sample(x, min(m, n))
+        ns = xs.shape[0]
+        d_x, d_lab = hip.malloc(xs.nbytes), hip.malloc(ns * 4)
+        hip.h2d(d_x, np.ascontiguousarray(xs))
+        p1 = dclust.make_params(dim, info["eps"], info["minpts"], precision=dclust.FP64)
+        hip.sync()
+        t0 = time.perf_counter()
+        st1 = h.run_onestep_device(d_x, ns, p1, d_lab)
+        dt = time.perf_counter() - t0
+        lab1 = np.empty(ns, dtype=np.int32)
+        hip.d2h(lab1, d_lab)
+        log(f"one-step baseline: {ns} points, {dt:.2f} s, {st1['n_passes']} launches")
+        if dt >= budget_s / 4 or ns >= n:
+            break
+        hip.free(d_x)
+        hip.free(d_lab)
+        m = int(m * min(8.0, max(1.5, budget_s / 2 / max(dt, 1e-3))))
+    p2 = dclust.make_params(dim, info["eps"], info["minpts"], r=info["partitions_r"],
+                            seeds=info["seed_list_size"], precision=prec)
+    for _ in range(2):
+        h.run_device(d_x, ns, p2, d_lab)
+    hip.sync()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        h.run_device(d_x, ns, p2, d_lab)
+    hip.sync()
+    main_s = (time.perf_counter() - t0) / reps
+    ref, core, _ = oracle.fast_dbscan(xs, info["eps"], info["minpts"])
+    core = core.astype(bool)
+    hip.free(d_x)
+    hip.free(d_lab)
+    return {"design": "CUDA-DClust one-step expansion (cuda-dclust/dbscan_indexing.cu): one "
+                      "seed per chain per launch, host read of the seed-list lengths after "
+                      "every launch, FP64, 256 blocks x 256 threads, 128 seeds",
+            "sample": f"{ns} points of the same input inside the sub-box [0,{a:.4f}]^{dim}",
+            "onestep_ms": round(dt * 1e3, 2), "onestep_mpoints_s": round(ns / dt / 1e6, 4),
+            "launches": st1["n_passes"], "chains": st1["n_chains"],
+            "main_path_ms": round(main_s * 1e3, 4),
+            "speedup_main_vs_onestep": round(dt / main_s, 1),
+            "core_label_mismatches": int(np.sum(lab1[core] != ref[core])),
+            "noise_mismatches": int(np.sum((lab1 == 0) != (ref == 0)))}
+
+
 BATCH_DESC = ("cfg4: 4,096 independent 2-D/3-D instances, sizes log-uniform 200-200,000, "
               "2-32 Gaussian blobs + 5% noise each, eps 0.5-2x the median 10-NN distance, "
               "minpts in {4,8,16}, FP32 + guard band")
@@ -287,6 +341,8 @@ def main():
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-vs-per-instance leg (cfg4)")
+    ap.add_argument("--no-onestep", action="store_true",
+                    help="skip the one-step (CUDA-DClust design) baseline leg")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a plain copy of known size, "
                          "used to calibrate the HBM byte counters of a profiler run)")
@@ -413,6 +469,9 @@ def main():
         log("batched leg")
         out["batched"] = batched_leg(hip, h, max(2, args.steps // 2), 1,
                                      check_oracle=not args.no_parity)
+    if not args.no_onestep:
+        log("one-step (CUDA-DClust) baseline")
+        out["onestep_baseline"] = onestep_leg(hip, h, x, info, prec)
     if not args.no_cpu_baseline:
         log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(x, info)

@@ -31,7 +31,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-I", INC, "-I", CSRC,
 ]
 
-HIP_SOURCES = ["api.hip", "index.hip", "cluster.hip"]
+HIP_SOURCES = ["api.hip", "index.hip", "cluster.hip", "onestep.hip"]
 HOST_SOURCES = ["io.cpp"]
 
 

@@ -102,7 +102,7 @@ enum Slot {
   S_G0, S_G1, S_G2, S_ORIG, S_CORE, S_LABEL, S_PARENT, S_COMPMIN, S_PAIRS, S_Q0, S_Q1,
   S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT, S_PAIRCNT, S_SEGPOS, S_LINES, S_AMASK,
   S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP, S_ADJPTS, S_CORECELLS, S_TUNELAB, S_STAT, S_GLINES,
-  S_CZ
+  S_CZ, S_OS_LABEL, S_OS_STATE, S_OS_SEEDS, S_OS_LEN, S_OS_CONTACT, S_OS_FIRST
 };
 
 // line table entries (8 bytes each) above which neighbour lines are found
@@ -388,9 +388,15 @@ void finish_run(dclust_handle* h, const Dev& d, const RunShape& shape, long long
 
 // One run over `ninst` instances whose coordinates are already on the
 // device (d_xyz, rows back to back) and whose labels go to d_labels.
+int onestep_loop(dclust_handle* h, const Dev& d, const dclust_params& prm, int threads,
+                 int32_t* d_labels, dclust_stats* stats);
+
+// onestep_threads > 0: the one-step baseline with workgroups of that size
+// runs after the index instead of the clustering phases
 int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64_t* n,
                const dclust_params* params, int32_t* d_labels, int32_t* inst_err,
-               dclust_stats* stats) {
+               dclust_stats* stats, int onestep_threads = 0) {
+  const bool onestep = onestep_threads > 0;
   hipError_t herr = hipSuccess;
   int rc = DCLUST_OK;
   if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -432,7 +438,12 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
   if (stats) stats->n_points = ntot;
   if (ntot == 0) return DCLUST_OK;
 
-  const RunShape shape = pick_shape(h, params[0], ntot, f32);
+  RunShape shape = pick_shape(h, params[0], ntot, f32);
+  if (onestep) {  // the expansion buffers stay unused: keep them minimal
+    shape.blocks = 8;
+    shape.waves = 8 * (shape.threads / 64);
+    shape.pair_cap = (int64_t)shape.pair_per * shape.waves;
+  }
   const int nchunks = (int)chunks.size();
   const int grid_cells = h->num_cu * 8;
   const hipStream_t s = h->stream;
@@ -523,7 +534,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
   // A call whose launch state equals the previous call's (sizes, parameters,
   // pointers) is captured once into a graph and replayed from then on.
   std::vector<uint8_t> sig;
-  const bool graph_on = graph_enabled(h);
+  const bool graph_on = !onestep && graph_enabled(h);
   if (graph_on) sig = launch_signature(d, shape, f32, inst, chunks);
   if (h->gexec && (!graph_on || sig != h->gsig)) drop_graph(h);
   if (graph_on && !h->gexec && !h->prev_sig.empty() && sig == h->prev_sig &&
@@ -601,6 +612,10 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
       d.line_cap = nlines;
     }
     launch_index_tail(d, ntot, nchunks, grid_cells, passes, s);
+    if (onestep) {
+      HIPCHK(mark(h, E_INDEX, false));
+      return onestep_loop(h, d, params[0], onestep_threads, d_labels, stats);
+    }
     HIPCHK(enqueue_core(h, d, shape, f32, false));
 
     // expansion passes until no seed is pending: pass p drains the queue of
@@ -633,6 +648,177 @@ fail:
   h->prev_sig.clear();
   return DCLUST_E_HIP;
 }
+
+// One-step baseline (CUDA-DClust): after the index, one launch per step of
+// every chain, the seed-list lengths read back after each launch, refills
+// when every list is empty, one union of the recorded contacts at the end.
+int onestep_loop(dclust_handle* h, const Dev& d, const dclust_params& prm, int threads,
+                 int32_t* d_labels, dclust_stats* stats) {
+#define OSCHK(x)                                 \
+  do {                                           \
+    const hipError_t e_ = (x);                   \
+    if (e_ != hipSuccess) {                      \
+      (void)hipGetLastError();                   \
+      if (stats) stats->hip_error = (int32_t)e_; \
+      h->last_err = (int)e_;                     \
+      return DCLUST_E_HIP;                       \
+    }                                            \
+  } while (0)
+  const hipStream_t s = h->stream;
+  const int64_t n = d.n;
+  const int B = prm.blocks > 0 ? prm.blocks : 256;
+  const int T = threads;
+  const int S = prm.seed_list_size > 0 ? prm.seed_list_size : 128;
+  hipError_t herr = hipSuccess;
+  OneStep o;
+  std::memset(&o, 0, sizeof(o));
+  o.blocks = B;
+  o.S = S;
+  // contacts are folded into the host union-find once half the list is
+  // used; one launch adds far fewer than the other half
+  o.cap = (int32_t)std::min<int64_t>(std::max<int64_t>(2 * n, 1 << 21), INT32_MAX / 2);
+  o.label = h->get<int32_t>(S_OS_LABEL, n, &herr);
+  o.state = h->get<int32_t>(S_OS_STATE, n, &herr);
+  o.seeds = h->get<int32_t>(S_OS_SEEDS, (size_t)B * S, &herr);
+  o.len = h->get<int32_t>(S_OS_LEN, (size_t)B + 2, &herr);  // lengths, then the counters
+  o.contacts = h->get<int2>(S_OS_CONTACT, o.cap, &herr);
+  int32_t* d_first = h->get<int32_t>(S_OS_FIRST, B, &herr);
+  if (herr != hipSuccess) return DCLUST_E_OOM;
+  o.ctr = o.len + B;
+
+  std::vector<int32_t> orig(n), inv(n), lab(n), len(B + 1), first(B), par;
+  std::vector<int2> ct;
+  int64_t launches = 0, rounds = 0, ncontacts = 0;
+  auto find = [&](int32_t x) {
+    while (par[x] != x) {
+      par[x] = par[par[x]];
+      x = par[x];
+    }
+    return x;
+  };
+  // the contacts recorded so far (cnt of them) into the union-find; the
+  // device counter restarts at 0
+  auto fold = [&](int32_t cnt) -> int {
+    if (cnt > o.cap) return DCLUST_E_CAPACITY;
+    ct.resize(cnt);
+    if (cnt > 0 &&
+        hipMemcpy(ct.data(), o.contacts, sizeof(int2) * cnt, hipMemcpyDeviceToHost) != hipSuccess)
+      return DCLUST_E_HIP;
+    const int32_t zero = 0;
+    if (hipMemcpy(o.ctr, &zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess)
+      return DCLUST_E_HIP;
+    for (const int2& e : ct) {
+      if (e.x < 0 || e.x >= (int64_t)par.size() || e.y < 0 || e.y >= (int64_t)par.size())
+        return DCLUST_E_CAPACITY;
+      const int32_t a = find(e.x), b = find(e.y);
+      if (a != b) par[std::max(a, b)] = std::min(a, b);
+    }
+    ncontacts += cnt;
+    return DCLUST_OK;
+  };
+  long long ctl_h[C_NCTL];
+  OSCHK(hipMemcpyAsync(ctl_h, d.ctl, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
+  OSCHK(hipMemcpyAsync(orig.data(), d.orig, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  launch_os_init(o, n, s);
+  OSCHK(hipStreamSynchronize(s));
+  if (ctl_h[C_ERR] || ctl_h[C_ABANDON]) return DCLUST_E_PARAM;
+  for (int64_t i = 0; i < n; ++i) inv[orig[i]] = (int32_t)i;
+  {
+    // refill seeds: unprocessed points from the back of the input order
+    int64_t cursor = n - 1;
+    const int64_t max_launches = 4 * n + 1024;
+    for (;;) {
+      OSCHK(hipMemcpyAsync(lab.data(), o.label, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+      OSCHK(hipStreamSynchronize(s));
+      int got = 0;
+      for (int b = 0; b < B; ++b) {
+        while (cursor >= 0 && lab[inv[cursor]] != -1) --cursor;
+        first[b] = cursor >= 0 ? inv[cursor--] : -1;
+        got += first[b] >= 0;
+      }
+      if (!got) break;
+      if ((rounds + 1) * (int64_t)B >= INT32_MAX) return DCLUST_E_CAPACITY;
+      OSCHK(hipMemcpyAsync(d_first, first.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, s));
+      launch_os_refill(o, d_first, s);
+      const int32_t chain0 = (int32_t)(rounds * B);
+      ++rounds;
+      const size_t old = par.size();
+      par.resize((size_t)rounds * B);
+      for (size_t c = old; c < par.size(); ++c) par[c] = (int32_t)c;
+      for (;;) {
+        launch_os_step(d, o, chain0, T, s);
+        ++launches;
+        OSCHK(hipMemcpyAsync(len.data(), o.len, sizeof(int32_t) * (B + 1), hipMemcpyDeviceToHost,
+                             s));
+        OSCHK(hipStreamSynchronize(s));
+        if (len[B] >= o.cap / 2) {
+          const int rc = fold(len[B]);
+          if (rc) return rc;
+        }
+        bool any = false;
+        for (int b = 0; b < B && !any; ++b) any = len[b] > 0;
+        if (!any) break;
+        if (launches > max_launches) return DCLUST_E_CAPACITY;
+      }
+    }
+  }
+  launch_os_border(d, o, s);
+  std::vector<int32_t> state(n);
+  int32_t nct = 0;
+  OSCHK(hipMemcpyAsync(lab.data(), o.label, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  OSCHK(hipMemcpyAsync(state.data(), o.state, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  OSCHK(hipMemcpyAsync(&nct, o.ctr, sizeof(nct), hipMemcpyDeviceToHost, s));
+  OSCHK(hipStreamSynchronize(s));
+  {
+    const int rc = fold(nct);
+    if (rc) return rc;
+  }
+
+  // canonical ids: a cluster is numbered by the rank of its smallest core
+  // input index
+  const int64_t nchains = rounds * B;
+  std::vector<int32_t> cmin(nchains, INT32_MAX);
+  int64_t ncore = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (state[i] == 0 || lab[i] == -1) return DCLUST_E_CAPACITY;  // never expanded
+    if (state[i] == 2) {
+      ++ncore;
+      const int32_t r = find(lab[i]);
+      cmin[r] = std::min(cmin[r], orig[i]);
+    }
+  }
+  std::vector<std::pair<int32_t, int32_t>> mins;
+  for (int64_t c = 0; c < nchains; ++c)
+    if (par[c] == c && cmin[c] != INT32_MAX) mins.push_back({cmin[c], (int32_t)c});
+  std::sort(mins.begin(), mins.end());
+  std::vector<int32_t> id(nchains, 0);
+  for (size_t k = 0; k < mins.size(); ++k) id[mins[k].second] = (int32_t)k + 1;
+  std::vector<int32_t> out(n);
+  int64_t nnoise = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t v = lab[i] >= 0 ? id[find(lab[i])] : 0;
+    out[orig[i]] = v;
+    nnoise += v == 0;
+  }
+  OSCHK(hipMemcpyAsync(d_labels, out.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  OSCHK(mark(h, E_END, false));
+  OSCHK(hipStreamSynchronize(s));
+  if (stats) {
+    stats->n_points = n;
+    stats->n_core = ncore;
+    stats->n_noise = nnoise;
+    stats->n_clusters = (int64_t)mins.size();
+    stats->n_chains = nchains;
+    stats->n_collisions = ncontacts;
+    stats->n_passes = launches;
+    stats->key_bits = (int32_t)ctl_h[C_KEYBITS];
+    stats->t_index_ms = ev_ms(h, E_START, E_INDEX);
+    stats->t_expand_ms = ev_ms(h, E_INDEX, E_END);
+    stats->t_total_ms = ev_ms(h, E_START, E_END);
+  }
+  return DCLUST_OK;
+}
+#undef OSCHK
 
 }  // namespace
 
@@ -706,6 +892,23 @@ int dclust_run_device(dclust_handle* h, const double* d_xyz, int64_t n, const dc
   if (!h || !p || n < 0 || (n > 0 && (!d_xyz || !d_labels_out))) return DCLUST_E_PARAM;
   if (hipSetDevice(h->device) != hipSuccess) return DCLUST_E_HIP;
   return run_device(h, 1, d_xyz, &n, p, d_labels_out, nullptr, stats);
+}
+
+int dclust_run_onestep_device(dclust_handle* h, const double* d_xyz, int64_t n,
+                              const dclust_params* p, int32_t* d_labels_out, dclust_stats* stats) {
+  if (!h || !p || n < 0 || (n > 0 && (!d_xyz || !d_labels_out))) return DCLUST_E_PARAM;
+  if (p->block_threads < 0 || p->block_threads > 1024 || p->block_threads % 64 != 0)
+    return DCLUST_E_PARAM;
+  dclust_params q = *p;
+  q.block_threads = 0;  // the main path's limit (<= 256) does not apply here
+  q.precision = DCLUST_PRECISION_FP64;
+  int c = check_params(&q);
+  if (c) return c;
+  if (p->blocks > (1 << 16)) return DCLUST_E_PARAM;
+  if (hipSetDevice(h->device) != hipSuccess) return DCLUST_E_HIP;
+  h->prev_sig.clear();
+  return run_device(h, 1, d_xyz, &n, &q, d_labels_out, nullptr, stats,
+                    p->block_threads > 0 ? p->block_threads : 256);
 }
 
 int dclust_run(dclust_handle* h, const double* xyz, int64_t n, const dclust_params* p,

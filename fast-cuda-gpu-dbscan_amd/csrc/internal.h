@@ -374,6 +374,18 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
   return (b & 7) * per + (b >> 3);
 }
 
+// State of the one-step (CUDA-DClust) baseline, cell-sorted order.
+struct OneStep {
+  int32_t* label;     // [n] -1 unclassified, -2 noise, >= 0 chain
+  int32_t* state;     // [n] 0 not expanded, 1 expanded, 2 expanded core
+  int32_t* seeds;     // [blocks * S] per-chain seed lists
+  int32_t* len;       // [blocks] seed-list lengths
+  int2* contacts;     // [cap] (chain, chain) contacts between core points
+  int32_t* ctr;       // [2] contacts recorded since the last host read (may
+                      // exceed cap: the rest were not stored), spare
+  int32_t blocks, S, cap, pad;
+};
+
 // The stencil lines of one cell: rows_of(d, c)[r] is line r (packed).
 struct RowsOf {
   const int32_t* base;

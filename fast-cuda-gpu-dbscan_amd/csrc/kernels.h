@@ -41,6 +41,14 @@ void launch_relabel(const Dev& d, int64_t n, hipStream_t s);
 void launch_core_out(const uint8_t* core, const int32_t* orig, int64_t n, uint8_t* out,
                      hipStream_t s);
 
+// one-step baseline: state reset, refill seeds (first[b] < 0: none), one
+// expansion step of every chain (chain ids chain0 + b), final border pass
+void launch_os_init(const OneStep& o, int64_t n, hipStream_t s);
+void launch_os_refill(const OneStep& o, const int32_t* first, hipStream_t s);
+void launch_os_step(const Dev& d, const OneStep& o, int32_t chain0, int threads,
+                    hipStream_t s);
+void launch_os_border(const Dev& d, const OneStep& o, hipStream_t s);
+
 // occupancy helper for the expansion kernel
 int expand_blocks_per_cu(int seeds, int threads);
 

@@ -101,6 +101,8 @@ def _lib():
         L.dclust_run_batched.argtypes = [_P, C.c_int32, _P, _P, _P, _P, _P, C.POINTER(Stats)]
         L.dclust_run_batched_device.argtypes = [_P, C.c_int32, _P, _P, _P, _P, _P,
                                                 C.POINTER(Stats)]
+        L.dclust_run_onestep_device.argtypes = [_P, _P, C.c_int64, C.POINTER(Params), _P,
+                                                C.POINTER(Stats)]
         L.dclust_last_core_flags.argtypes = [_P, _P, C.c_int64]
         L.dclust_autotune.argtypes = [_P, C.c_int64, C.POINTER(Params)]
         L.dclust_autotune_device.argtypes = [_P, C.c_int32, _P, _P, _P]
@@ -213,6 +215,16 @@ class Handle:
         _check(_lib().dclust_run_device(self._h, C.c_void_p(d_xyz), int(n), C.byref(params),
                                         C.c_void_p(d_labels), C.byref(st)),
                "dclust_run_device")
+        self.last_stats = st.as_dict()
+        return self.last_stats
+
+    def run_onestep_device(self, d_xyz: int, n: int, params: Params, d_labels: int):
+        """The one-step (CUDA-DClust) baseline on device buffers."""
+        st = Stats()
+        _check(_lib().dclust_run_onestep_device(self._h, C.c_void_p(d_xyz), int(n),
+                                                C.byref(params), C.c_void_p(d_labels),
+                                                C.byref(st)),
+               "dclust_run_onestep_device")
         self.last_stats = st.as_dict()
         return self.last_stats
 

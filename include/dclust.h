@@ -159,6 +159,24 @@ int dclust_run_device(dclust_handle* h, const double* d_xyz, int64_t n,
                       const dclust_params* p, int32_t* d_labels_out,
                       dclust_stats* stats);
 
+/* One-step expansion baseline: CUDA-DClust as the reference ships it
+ * (cuda-dclust/dbscan_indexing.cu: kernel DBSCAN :529-637, host loop
+ * :442-472, MonitorSeedPoints :639-781), on this library's grid index.
+ * One seed per chain per launch, a host read of the seed-list lengths
+ * after every launch, new seeds only when every list is empty.  It is here
+ * to time the design the main path replaces, not as a clustering path of
+ * its own.  Parameters: blocks = THREAD_BLOCKS (0: 256), block_threads =
+ * THREAD_COUNT (0: 256; a multiple of 64 up to 1024), seed_list_size =
+ * MAX_SEEDS (0: 128); the FP64 predicate whatever `precision` says.
+ * Labels: core points and noise exactly as dclust_run_device gives them; a
+ * border point gets the cluster of some core point within eps (not always
+ * the smallest id).  Device pointers, one instance, no graph replay.
+ * Stats: n_passes = kernel launches, n_chains = chains started,
+ * n_collisions = contacts recorded, t_expand_ms = launches to labels. */
+int dclust_run_onestep_device(dclust_handle* h, const double* d_xyz, int64_t n,
+                              const dclust_params* p, int32_t* d_labels_out,
+                              dclust_stats* stats);
+
 /* Cluster `count` independent instances in one pass over the device
  * (no counterpart in the reference, which clusters one dataset per process).
  * xyz[i]: n[i] rows of params[i].dim doubles on the host; labels[i]: n[i]

@@ -53,3 +53,15 @@ def test_struct_layout_matches_header():
     assert ctypes.sizeof(dclust.Params) == 40
     assert dclust.Params.eps.offset == 8
     assert ctypes.sizeof(dclust.Stats) == 9 * 8 + 17 * 8 + 2 * 4 + 16 * 8
+
+
+def test_device_entry_points_reject_bad_arguments_before_any_gpu_call():
+    lib = ctypes.CDLL(dclust.LIB_PATH)
+    P = ctypes.c_void_p
+    p = dclust.make_params(2, 1.5, 3)
+    for fn in (lib.dclust_run_device, lib.dclust_run_onestep_device):
+        fn.argtypes = [P, P, ctypes.c_int64, ctypes.POINTER(dclust.Params), P, P]
+        # no handle, no parameters, a negative count
+        assert fn(None, None, 0, ctypes.byref(p), None, None) == dclust.E_PARAM
+        assert fn(P(1), None, 0, None, None, None) == dclust.E_PARAM
+        assert fn(P(1), None, -1, ctypes.byref(p), None, None) == dclust.E_PARAM
