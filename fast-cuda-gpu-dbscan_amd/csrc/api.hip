@@ -677,6 +677,11 @@ int onestep_loop(dclust_handle* h, const Dev& d, const dclust_params& prm, int t
   // contacts are folded into the host union-find once half the list is
   // used; one launch adds far fewer than the other half
   o.cap = (int32_t)std::min<int64_t>(std::max<int64_t>(2 * n, 1 << 21), INT32_MAX / 2);
+  // a smaller list (a test knob: folds happen every few launches)
+  if (const char* e = std::getenv("DCLUST_ONESTEP_CONTACTS")) {
+    const long v = std::atol(e);
+    if (v >= 64 && v < o.cap) o.cap = (int32_t)v;
+  }
   o.label = h->get<int32_t>(S_OS_LABEL, n, &herr);
   o.state = h->get<int32_t>(S_OS_STATE, n, &herr);
   o.seeds = h->get<int32_t>(S_OS_SEEDS, (size_t)B * S, &herr);

@@ -172,7 +172,10 @@ int dclust_run_device(dclust_handle* h, const double* d_xyz, int64_t n,
  * border point gets the cluster of some core point within eps (not always
  * the smallest id).  Device pointers, one instance, no graph replay.
  * Stats: n_passes = kernel launches, n_chains = chains started,
- * n_collisions = contacts recorded, t_expand_ms = launches to labels. */
+ * n_collisions = contacts recorded, t_expand_ms = launches to labels.
+ * Contacts go to a device list of max(2n, 2^21) entries that the host folds
+ * into its union-find whenever half is used; DCLUST_E_CAPACITY if one
+ * launch records more than the other half. */
 int dclust_run_onestep_device(dclust_handle* h, const double* d_xyz, int64_t n,
                               const dclust_params* p, int32_t* d_labels_out,
                               dclust_stats* stats);

@@ -96,3 +96,17 @@ def test_onestep_full_lists_and_shapes(handle, devmem, seeds, blocks, threads):
 def test_onestep_bad_shape(handle, devmem):
     with pytest.raises(dclust.DclustError):
         _onestep(handle, devmem, EX, 1.5, 3, threads=100)
+
+
+def test_onestep_contacts_folded_during_the_run(handle, devmem, monkeypatch):
+    # a 4096-entry contact list fills many times over: the host folds it into
+    # its union-find whenever half is used, and the labels stay exact (32
+    # chains keep one launch's contacts below the 2048 entries left)
+    monkeypatch.setenv("DCLUST_ONESTEP_CONTACTS", "4096")
+    rng = np.random.default_rng(21)
+    n = 20000
+    centres = rng.uniform(0, 1, (6, 3))
+    x = centres[rng.integers(0, 6, n)] + rng.normal(0, 0.03, (n, 3))
+    lab, st = _onestep(handle, devmem, x, 0.02, 5, seeds=4, blocks=32)
+    assert st["n_collisions"] > 4096
+    _verify(x, 0.02, 5, lab, st)
