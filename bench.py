@@ -12,8 +12,12 @@ over the timed steps.
 
 Also reported (not timed in `value`): the plain FP64 form's time on the same
 input, label/core-flag parity against the FP64 CPU oracle, the neighbour-
-count kernel's roofline fraction, and the CPU baseline (the oracle's
-sequential FP64 DBSCAN on a bounded spatial sample of the same points).
+count kernel's roofline fraction, the batched leg (4,096 instances in one
+call against one call per instance), the one-step GPU baseline (the
+CUDA-DClust design on a bounded spatial sample, with the main path timed on
+the same sample), and two CPU baselines on bounded spatial samples of the
+same points: the oracle's sequential FP64 DBSCAN (1 thread) and the
+multicore-cpu restatement (up to 16 threads).
 """
 from __future__ import annotations
 
@@ -350,7 +354,7 @@ def main():
     ws = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or ws != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
-              "(the sharded design is described in DESIGN.md).", file=sys.stderr)
+              "(how the path would shard is written up with the design notes).", file=sys.stderr)
         sys.exit(2)
 
     cfg, desc = WORKLOADS[args.workload]
