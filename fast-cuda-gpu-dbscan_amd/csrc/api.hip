@@ -102,7 +102,7 @@ enum Slot {
   S_G0, S_G1, S_G2, S_ORIG, S_CORE, S_LABEL, S_PARENT, S_COMPMIN, S_PAIRS, S_Q0, S_Q1,
   S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT, S_PAIRCNT, S_SEGPOS, S_LINES, S_AMASK,
   S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP, S_ADJPTS, S_CORECELLS, S_TUNELAB, S_STAT, S_GLINES,
-  S_CZ, S_OS_LABEL, S_OS_STATE, S_OS_SEEDS, S_OS_LEN, S_OS_CONTACT, S_OS_FIRST
+  S_CZ, S_XOFF, S_OS_LABEL, S_OS_STATE, S_OS_SEEDS, S_OS_LEN, S_OS_CONTACT, S_OS_FIRST
 };
 
 // line table entries (8 bytes each) above which neighbour lines are found
@@ -503,9 +503,13 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     d.cell_ncore = h->get<int32_t>(S_CM, ntot, &E);
     d.rows = h->get<int32_t>(S_ROWS, (size_t)ntot * kMaxRows, &E);
     d.cell_of = h->get<int32_t>(S_CELLOF, ntot, &E);
-    d.X[0] = h->get<double>(S_X0, ntot, &E);
-    d.X[1] = h->get<double>(S_X1, ntot, &E);
-    d.X[2] = h->get<double>(S_X2, ntot, &E);
+    if (!f32) {  // the FP32 form reads FP64 coordinates from the caller's rows
+      d.X[0] = h->get<double>(S_X0, ntot, &E);
+      d.X[1] = h->get<double>(S_X1, ntot, &E);
+      d.X[2] = h->get<double>(S_X2, ntot, &E);
+    } else {
+      d.xoff = h->get<int64_t>(S_XOFF, ntot, &E);
+    }
     d.G[0] = h->get<uint32_t>(S_G0, ntot, &E);
     d.G[1] = h->get<uint32_t>(S_G1, ntot, &E);
     d.G[2] = h->get<uint32_t>(S_G2, ntot, &E);

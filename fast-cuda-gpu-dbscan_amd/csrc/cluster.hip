@@ -125,8 +125,8 @@ __device__ __forceinline__ bool edge_pair(const Dev& d, const Par& P, int32_t a,
     if (s < P.d2_in) return true;
     if (s > P.d2_out) return false;
     t.band++;
-    load_x<D>(d, a, xa);
-    load_x<D>(d, b, xb);
+    load_x<D, F32>(d, a, xa);
+    load_x<D, F32>(d, b, xb);
   } else {
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -208,7 +208,7 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
   if (F32) {
     load_g<D>(d, p, pg);
   } else {
-    load_x<D>(d, p, px);
+    load_x<D, F32>(d, p, px);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       lo[k] = px[k] - P.eps;
@@ -236,7 +236,7 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
         if (F32)
           load_g<D>(d, qi[j], cg[j]);
         else
-          load_x<D>(d, qi[j], cx[j]);
+          load_x<D, F32>(d, qi[j], cx[j]);
       }
 #pragma unroll
       for (int j = 0; j < B; ++j) {
@@ -251,8 +251,8 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
               fwd = sym = false;
             } else {
               double ax[D], bx[D];
-              load_x<D>(d, p, ax);
-              load_x<D>(d, q, bx);
+              load_x<D, F32>(d, p, ax);
+              load_x<D, F32>(d, q, bx);
               const double dd = d2_64<D>(ax, bx);
               sym = dd <= P.eps2;
               fwd = sym && box_64<D>(ax, bx, P.eps);
@@ -1370,13 +1370,13 @@ __device__ __forceinline__ bool pred_to(const Dev& d, const Par& P, int32_t q, i
     if (s > P.d2_out) return false;
     t.band++;
     double ax[D];
-    load_x<D>(d, q, qx);
-    load_x<D>(d, p, ax);
+    load_x<D, F32>(d, q, qx);
+    load_x<D, F32>(d, p, ax);
     const bool ex = box_64<D>(qx, ax, P.eps) && d2_64<D>(qx, ax) <= P.eps2;
     t.flip += ex != (s <= P.d2_mid);
     return ex;
   }
-  load_x<D>(d, q, qx);
+  load_x<D, F32>(d, q, qx);
   return box_64<D>(qx, px, P.eps) && d2_64<D>(qx, px) <= P.eps2;
 }
 
@@ -1398,7 +1398,7 @@ __device__ int32_t border_lane(const Dev& d, const Par& P, int32_t p, int32_t c,
   if (F32)
     load_g<D>(d, p, pg);
   else
-    load_x<D>(d, p, px);
+    load_x<D, F32>(d, p, px);
   const int n = R > 0 ? (int)d.rec_n[p] : 256;
   if (n <= R) {
     const int32_t* rec = d.rec + (int64_t)p * R;

@@ -619,10 +619,12 @@ __global__ __launch_bounds__(256) void k_gather(Dev d) {
     const int32_t src = vsrc[i];
     const Inst& I = *inst_of_cell(d, d.ninst == 1 ? 0 : d.cell_of[i]);
     const int dim = I.dim;
-    const double* p = d.xyz + I.xyz_off + ((int64_t)src - I.pt_begin) * dim;
+    const int64_t off = I.xyz_off + ((int64_t)src - I.pt_begin) * dim;
+    const double* p = d.xyz + off;
+    if (!d.X[0]) d.xoff[i] = off;  // the FP32 form's way back to FP64 rows
     for (int k = 0; k < dim; ++k) {
       double x = p[k];
-      d.X[k][i] = x;
+      if (d.X[0]) d.X[k][i] = x;  // the FP64 form's sorted copy
       double t = cell_t(x, I.mn[k], I.inv_w);       // same rounding as the key
       double f = floor(t * 0x1p29);                  // exact scaling by 2^29
       d.G[k][i] = (uint32_t)(unsigned long long)(long long)f;  // wraps mod 8 cells

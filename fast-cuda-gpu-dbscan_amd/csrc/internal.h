@@ -7,7 +7,7 @@
 //   CSR     cell_start[C+1], cell_key[C], cell_inst[C], rows[C][25] (packed
 //           runs of the 5^(D-1) neighbour lines, centre line first),
 //           cell_ncore[C], cell_label[C] (chain), amask[C] (near adjacency)
-//   points  (cell-sorted)  X[3][N] FP64 SoA, G[3][N] u32 fixed-point
+//   points  (cell-sorted)  X[3][N] FP64 SoA (FP64 form only), G[3][N] u32 fixed-point
 //           coordinates (29 fraction bits per cell, wrapping mod 8 cells),
 //           orig[N], cell_of[N], core[N] u8, rec[N][R] + rec_n[N] (the
 //           neighbours of each non-core point)
@@ -210,7 +210,8 @@ struct Dev {
   int32_t nbr_stats;    // 1: k_rows also sums the neighbourhood sizes
 
   // sorted point data
-  double* X[3];
+  double* X[3];         // FP64 form only (null in the FP32 form)
+  int64_t* xoff;        // FP32 form: offset of each point's row in xyz
   uint32_t* G[3];
   int32_t* orig;
   uint8_t* core;

@@ -110,7 +110,7 @@ __device__ void os_step(const Dev& d, const OneStep& o, int32_t chain) {
   const double eps = I.eps, eps2 = I.eps2;
   constexpr int NR = D == 3 ? 25 : 5;
   double px[D];
-  load_x<D>(d, p, px);
+  load_x<D, false>(d, p, px);
   const RowsOf rows = rows_of(d, d.cell_of[p]);
 
   // neighbour count of p, self included (dbscan_indexing.cu:581-617)
@@ -124,7 +124,7 @@ __device__ void os_step(const Dev& d, const OneStep& o, int32_t chain) {
       int jr;
       const int32_t q = st.at(i, jr);
       double qx[D];
-      load_x<D>(d, q, qx);
+      load_x<D, false>(d, q, qx);
       cnt += box_64<D>(px, qx, eps) && d2_64<D>(px, qx) <= eps2;
     }
   }
@@ -157,7 +157,7 @@ __device__ void os_step(const Dev& d, const OneStep& o, int32_t chain) {
       const int32_t q = st.at(i, jr);
       if (q == p) continue;
       double qx[D];
-      load_x<D>(d, q, qx);
+      load_x<D, false>(d, q, qx);
       if (!(box_64<D>(px, qx, eps) && d2_64<D>(px, qx) <= eps2)) continue;
       const int32_t lq = ld_agent(o.label + q);
       if (lq == -1) {
@@ -200,7 +200,7 @@ __device__ void os_border(const Dev& d, const OneStep& o, int32_t p) {
   const Inst& I = *d.inst;
   constexpr int NR = D == 3 ? 25 : 5;
   double px[D];
-  load_x<D>(d, p, px);
+  load_x<D, false>(d, p, px);
   const RowsOf rows = rows_of(d, d.cell_of[p]);
   for (int g = 0; g < NR; g += 5) {
     int32_t aa[5], ll[5];
@@ -212,7 +212,7 @@ __device__ void os_border(const Dev& d, const OneStep& o, int32_t p) {
       const int32_t q = st.at(i, jr);
       if (o.state[q] != 2) continue;
       double qx[D];
-      load_x<D>(d, q, qx);
+      load_x<D, false>(d, q, qx);
       if (box_64<D>(qx, px, I.eps) && d2_64<D>(qx, px) <= I.eps2) {
         o.label[p] = o.label[q];
         return;

@@ -7,10 +7,19 @@
 namespace dclust {
 
 // ------------------------------------------------------------ predicates --
-template <int D>
+// FP64 coordinates of cell-sorted point i.  The FP32 form keeps no sorted
+// FP64 copy (only its guard-band rechecks need FP64): they are read from
+// the caller's rows at the point's row offset.
+template <int D, bool F32>
 __device__ __forceinline__ void load_x(const Dev& d, int32_t i, double* x) {
+  if constexpr (F32) {
+    const double* p = d.xyz + d.xoff[i];
 #pragma unroll
-  for (int k = 0; k < D; ++k) x[k] = d.X[k][i];
+    for (int k = 0; k < D; ++k) x[k] = p[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < D; ++k) x[k] = d.X[k][i];
+  }
 }
 template <int D>
 __device__ __forceinline__ void load_g(const Dev& d, int32_t i, uint32_t* g) {
