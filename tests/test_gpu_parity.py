@@ -448,3 +448,38 @@ def test_graph_replay_reruns_when_the_key_widens(handle, prec, monkeypatch):
         out.append((st["diag"][14], st["key_bits"]))
     assert [r for r, _ in out] == [0, 1, 0, 1, 1]
     assert out[1][1] <= 8 and out[2][1] > 16
+
+
+def test_point_count_limit(handle, devmem):
+    # cell indices are packed into 28 bits: 2^28 points are refused on the
+    # host, before any buffer is touched
+    p = dclust.make_params(3, 0.01, 4)
+    d_x, d_lab = devmem.alloc(64), devmem.alloc(64)
+    for run in (handle.run_device, handle.run_onestep_device):
+        with pytest.raises(dclust.DclustError) as e:
+            run(d_x, 1 << 28, p, d_lab)
+        assert e.value.code == dclust.E_CAPACITY
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_axis_limit_3d(handle, prec):
+    # a 3-D axis holds at most 2^21 cells of width eps/2: 5000 / 0.001 is more
+    rng = np.random.default_rng(91)
+    x = rng.uniform(0, 1, (2000, 3))
+    x[0] = [5000.0, 0.5, 0.5]
+    with pytest.raises(dclust.DclustError) as e:
+        handle.run(x, 0.002, 4, precision=prec)
+    assert e.value.code == dclust.E_CAPACITY
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_wide_2d_extent(handle, prec):
+    # two groups 10^6 apart with eps 10^-3: about 2*10^9 cells on x (below the
+    # 2-D limit of 2^31), a cell key wider than 32 bits (five radix passes) and
+    # no line table
+    rng = np.random.default_rng(92)
+    a = rng.normal(0.5, 0.01, (1500, 2))
+    b = rng.normal(0.5, 0.01, (1500, 2)) + np.array([1.0e6, 0.0])
+    x = np.concatenate([a, b])[rng.permutation(3000)]
+    st = _check(handle, x, 0.001, 4, prec)
+    assert st["key_bits"] > 32
