@@ -483,3 +483,29 @@ def test_wide_2d_extent(handle, prec):
     x = np.concatenate([a, b])[rng.permutation(3000)]
     st = _check(handle, x, 0.001, 4, prec)
     assert st["key_bits"] > 32
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_batched_ragged_with_empty_instances(handle, prec):
+    # empty, one-point and two-point instances between ordinary ones: each
+    # instance is clustered as if alone, the empty ones give empty labels
+    rng = np.random.default_rng(93)
+    xs, params = _batch(limit=12, max_n=20000)
+    inst = [
+        (np.empty((0, 2)), 0.1, 3),
+        (np.array([[0.3, 0.4, 0.5]]), 0.1, 1),
+        (xs[0], params[0].eps, params[0].minpts),
+        (np.empty((0, 3)), 0.05, 4),
+        (np.array([[0.0, 0.0], [5.0, 5.0]]), 0.5, 1),
+        (rng.uniform(0, 1, (3000, 3)), 0.05, 5),
+        (xs[1], params[1].eps, params[1].minpts),
+    ]
+    plist = [dclust.make_params(x.shape[1], eps, mp, precision=prec) for x, eps, mp in inst]
+    labs, errs = handle.run_batched([x for x, _, _ in inst], plist)
+    assert errs.tolist() == [0] * len(inst)
+    for i, (x, eps, mp) in enumerate(inst):
+        assert labs[i].shape == (x.shape[0],)
+        if x.shape[0]:
+            ref, _, asym = oracle.fast_dbscan(x, eps, mp)
+            assert asym == 0 and np.array_equal(labs[i], ref), f"instance {i}"
+    assert labs[1].tolist() == [1] and labs[4].tolist() == [1, 2]
