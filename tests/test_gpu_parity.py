@@ -509,3 +509,27 @@ def test_batched_ragged_with_empty_instances(handle, prec):
             ref, _, asym = oracle.fast_dbscan(x, eps, mp)
             assert asym == 0 and np.array_equal(labs[i], ref), f"instance {i}"
     assert labs[1].tolist() == [1] and labs[4].tolist() == [1, 2]
+
+
+def test_guard_band_decides_pairs_fp32_cannot_resolve(handle):
+    # 1000 isolated pairs at distance eps*(1 + d), |d| <= 3e-8: below FP32's
+    # resolution, so every pair falls in the guard band and is re-decided in
+    # FP64; with minpts 2 a pair is a cluster exactly when the FP64 predicate
+    # holds.  Some FP32 decisions must be flipped, and the labels equal the
+    # oracle's (and the FP64 form's).
+    rng = np.random.default_rng(94)
+    eps = 1e-3
+    g = np.stack(np.meshgrid(*[np.arange(10)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    a = 0.05 + g * 0.01
+    u = rng.normal(size=(1000, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    d = rng.uniform(-3e-8, 3e-8, (1000, 1))
+    b = a + u * eps * (1.0 + d)
+    x = np.concatenate([a, b])[rng.permutation(2000)]
+    ref, _, asym = oracle.fast_dbscan(x, eps, 2)
+    assert asym == 0 and 200 < ref.max() < 800  # both outcomes occur
+    lab32 = handle.run(x, eps, 2, precision=dclust.FP32_GUARD)
+    st = handle.last_stats
+    assert np.array_equal(lab32, ref)
+    assert st["band_rechecks"] >= 1000 and st["band_flips"] > 0
+    assert np.array_equal(handle.run(x, eps, 2, precision=dclust.FP64), ref)
