@@ -18,6 +18,7 @@
 #include "internal.h"
 #include "kernels.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace dclust {
@@ -854,10 +855,16 @@ void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
   hipLaunchKernelGGL(k_csr_count, dim3(ncs), dim3(CS_T), 0, s, d);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, d.scan_tmp, ncs, (long long*)nullptr);
   hipLaunchKernelGGL(k_csr_write, dim3(ncs), dim3(CS_T), 0, s, d);
-  hipLaunchKernelGGL(k_gather, dim3(fill_grid(n)), dim3(256), 0, s, d);
+  // random-access gather: more blocks in flight than a fill needs
+  const int64_t gb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_gather, dim3((int)std::max<int64_t>(1, std::min<int64_t>(16384, gb))),
+                     dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_line_clear, dim3(grid_cells), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_line_mark, dim3(grid_cells), dim3(256), 0, s, d);
-  hipLaunchKernelGGL(k_rows, dim3(grid_cells, 5), dim3(256), 0, s, d);
+  // 4 x grid_cells workgroups per line group, but no more than the cells
+  // (at most n) need: the dependent searches want many waves in flight
+  const int rb = (int)std::max<int64_t>(1, std::min<int64_t>(4LL * grid_cells, gb));
+  hipLaunchKernelGGL(k_rows, dim3(rb, 5), dim3(256), 0, s, d);
 }
 
 void launch_scan_top(int64_t* v, int nb, hipStream_t s) {
