@@ -303,10 +303,12 @@ __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
 }
 
 // ------------------------------------------------------------ radix sort --
-// LSD radix sort, 8-bit digits, 4096-key tiles (256 threads x 16 keys).
-// Buffer parity: the number of scatters executed so far.
+// LSD radix sort, 8-bit digits, 2048-key tiles (256 threads x 8 keys: the
+// scatter's 24 KiB of LDS lets six workgroups share a CU; 4096-key tiles
+// were slower, 1024-key tiles too).  Buffer parity: the number of scatters
+// executed so far.
 constexpr int RS_T = 256;
-constexpr int RS_I = 16;
+constexpr int RS_I = 8;
 constexpr int RS_TILE = RS_T * RS_I;
 
 __device__ __forceinline__ int sort_parity(const Dev& d, int pass) {
