@@ -483,10 +483,10 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) 
 
 // ------------------------------------------------------- CSR compaction --
 // Flag = sorted key differs from its predecessor.  Pass 1 counts flags per
-// 4096-key tile with ballot/popcount; a single block scans the tile counts;
+// 2048-key tile with ballot/popcount; a single block scans the tile counts;
 // pass 2 ranks flags inside the tile with mbcnt and writes the cell ranges.
 constexpr int CS_T = 256;
-constexpr int CS_I = 16;
+constexpr int CS_I = 8;
 constexpr int CS_TILE = CS_T * CS_I;
 
 __device__ __forceinline__ const uint64_t* sorted_keys(const Dev& d) {
