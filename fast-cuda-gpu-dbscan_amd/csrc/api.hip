@@ -119,9 +119,10 @@ int check_params(const dclust_params* p) {
   if (p->partitions_r < 0 || p->seed_list_size < 0 || p->blocks < 0 || p->block_threads < 0 ||
       p->extra_collision < 0)
     return DCLUST_E_PARAM;
-  if (p->block_threads != 0 && p->block_threads != 64 && p->block_threads != 128 &&
-      p->block_threads != 256)
-    return DCLUST_E_PARAM;
+  // THREAD_COUNT: any workgroup size up to 1024 is accepted (common.h:14
+  // uses 16); the expansion runs it rounded up to whole wavefronts, at most
+  // 256 threads (effective_threads)
+  if (p->block_threads > 1024) return DCLUST_E_PARAM;
   // seed lists live in LDS: (threads / 64) * seeds * 4 bytes <= 64 KiB
   if (p->seed_list_size > 4096) return DCLUST_E_PARAM;
   if (p->precision != DCLUST_PRECISION_FP64 && p->precision != DCLUST_PRECISION_FP32_GUARD)
@@ -148,12 +149,21 @@ struct RunShape {
   int64_t pair_cap;
 };
 
+// Workgroup size the expansion runs for a THREAD_COUNT of t (0 = auto):
+// rounded up to whole wavefronts of 64 lanes (each wavefront owns a seed
+// list, so a partial wavefront would idle lanes the hardware runs anyway),
+// at most 256 (the kernel's launch bound).
+int effective_threads(int t) {
+  if (t <= 0) return 256;
+  return std::min(256, (t + 63) / 64 * 64);
+}
+
 // Launch shape: explicit parameters win; zeros come from the tuning rule
 // (seed list 1024 entries, 256 threads, blocks at resident occupancy).
 RunShape pick_shape(dclust_handle* h, const dclust_params& p, int64_t n, bool f32) {
   RunShape r;
   r.seeds = p.seed_list_size > 0 ? p.seed_list_size : 1024;
-  r.threads = p.block_threads > 0 ? p.block_threads : 256;
+  r.threads = effective_threads(p.block_threads);
   if (p.blocks > 0) {
     r.blocks = p.blocks;
   } else {
@@ -321,6 +331,14 @@ hipError_t read_inst_errors(const Dev& d, int32_t ninst, int32_t* inst_err, int*
   return hipSuccess;
 }
 
+// The launch shape a run used, as the common.h names count it.
+void set_shape_stats(const RunShape& shape, dclust_stats* stats) {
+  stats->eff_blocks = shape.blocks;
+  stats->eff_block_threads = shape.threads;
+  stats->eff_seed_list_size = shape.seeds;
+  stats->eff_extra_collision = shape.pair_per * (shape.threads / 64);
+}
+
 // Records the run's device state for dclust_last_core_flags and fills the
 // statistics (counters summed over their stripes, phase times from events).
 void finish_run(dclust_handle* h, const Dev& d, const RunShape& shape, long long* ctl_h,
@@ -375,6 +393,7 @@ void finish_run(dclust_handle* h, const Dev& d, const RunShape& shape, long long
   stats->diag[13] = ctl_h[C_SEEDS_MAX];
   stats->diag[14] = replayed ? 1 : 0;
   stats->diag[15] = ctl_h[C_ROUNDS_MAX];
+  set_shape_stats(shape, stats);
 }
 
 #define HIPCHK(x)                          \
@@ -599,6 +618,12 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     enqueue_head(d, nchunks, s);
     HIPCHK(hipMemcpyAsync(ctl_h, d.ctl, sizeof(ctl_h), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    // more cells than the 63-bit sort key holds, over all instances
+    if (ctl_h[C_KEYOVF]) {
+      if (inst_err)
+        for (int32_t i = 0; i < ninst; ++i) inst_err[i] = DCLUST_E_CAPACITY;
+      return DCLUST_E_CAPACITY;
+    }
     if (ctl_h[C_ERR]) {
       int first = 0;
       HIPCHK(read_inst_errors(d, ninst, inst_err, &first));
@@ -821,6 +846,10 @@ int onestep_loop(dclust_handle* h, const Dev& d, const dclust_params& prm, int t
     stats->n_collisions = ncontacts;
     stats->n_passes = launches;
     stats->key_bits = (int32_t)ctl_h[C_KEYBITS];
+    stats->eff_blocks = B;
+    stats->eff_block_threads = T;
+    stats->eff_seed_list_size = S;
+    stats->eff_extra_collision = 0;
     stats->t_index_ms = ev_ms(h, E_START, E_INDEX);
     stats->t_expand_ms = ev_ms(h, E_INDEX, E_END);
     stats->t_total_ms = ev_ms(h, E_START, E_END);
@@ -840,10 +869,10 @@ int dclust_params_default(dclust_params* p) {
   p->eps = 1.5;      // EPS
   p->minpts = 3;     // MINPTS
   p->partitions_r = 3;  // PARTITION_SIZE
-  p->seed_list_size = 0;
-  p->blocks = 0;
-  p->block_threads = 0;
-  p->extra_collision = 0;
+  p->seed_list_size = 128;    // MAX_SEEDS
+  p->blocks = 8;              // THREAD_BLOCKS
+  p->block_threads = 16;      // THREAD_COUNT (runs as one wavefront of 64)
+  p->extra_collision = 256;   // EXTRA_COLLISION_SIZE
   p->precision = DCLUST_PRECISION_FP64;
   return DCLUST_OK;
 }
@@ -906,8 +935,7 @@ int dclust_run_device(dclust_handle* h, const double* d_xyz, int64_t n, const dc
 int dclust_run_onestep_device(dclust_handle* h, const double* d_xyz, int64_t n,
                               const dclust_params* p, int32_t* d_labels_out, dclust_stats* stats) {
   if (!h || !p || n < 0 || (n > 0 && (!d_xyz || !d_labels_out))) return DCLUST_E_PARAM;
-  if (p->block_threads < 0 || p->block_threads > 1024 || p->block_threads % 64 != 0)
-    return DCLUST_E_PARAM;
+  if (p->block_threads < 0 || p->block_threads > 1024) return DCLUST_E_PARAM;
   dclust_params q = *p;
   q.block_threads = 0;  // the main path's limit (<= 256) does not apply here
   q.precision = DCLUST_PRECISION_FP64;
@@ -916,8 +944,9 @@ int dclust_run_onestep_device(dclust_handle* h, const double* d_xyz, int64_t n,
   if (p->blocks > (1 << 16)) return DCLUST_E_PARAM;
   if (hipSetDevice(h->device) != hipSuccess) return DCLUST_E_HIP;
   h->prev_sig.clear();
-  return run_device(h, 1, d_xyz, &n, &q, d_labels_out, nullptr, stats,
-                    p->block_threads > 0 ? p->block_threads : 256);
+  // whole wavefronts, as THREAD_COUNT is counted on the main path
+  const int threads = p->block_threads > 0 ? (p->block_threads + 63) / 64 * 64 : 256;
+  return run_device(h, 1, d_xyz, &n, &q, d_labels_out, nullptr, stats, threads);
 }
 
 int dclust_run(dclust_handle* h, const double* xyz, int64_t n, const dclust_params* p,

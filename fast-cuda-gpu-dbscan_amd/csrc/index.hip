@@ -148,9 +148,10 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d) {
 // cells sharing all coordinates but the last; the dense line table (one
 // entry per possible line) locates a neighbour line in O(1).
 __global__ void k_grid_setup(Dev d) {
-  __shared__ long long s_scan[1024];
+  __shared__ unsigned long long s_scan[1024];
   __shared__ long long s_scanl[1024];
-  __shared__ long long s_carry, s_carryl;
+  __shared__ unsigned long long s_carry;
+  __shared__ long long s_carryl;
   __shared__ int s_err;
   if (threadIdx.x == 0) {
     s_carry = 0;
@@ -160,7 +161,7 @@ __global__ void k_grid_setup(Dev d) {
   __syncthreads();
   for (int base = 0; base < d.ninst; base += blockDim.x) {
     const int i = base + threadIdx.x;
-    long long cells = 0;
+    unsigned long long cells = 0;  // at most 2^63 (3 axes of at most 2^21 cells)
     long long lines = 0;
     if (i < d.ninst) {
       Inst I = d.inst[i];
@@ -224,7 +225,7 @@ __global__ void k_grid_setup(Dev d) {
         if (k < dim) {
           double t = floor(cell_t(mx[k], mn[k], I.inv_w));
           I.nk[k] = (int32_t)t + 1;
-          cells *= (long long)I.nk[k];
+          cells *= (unsigned long long)I.nk[k];
           if (k < dim - 1) lines *= (long long)I.nk[k];
         }
       }
@@ -253,32 +254,40 @@ __global__ void k_grid_setup(Dev d) {
     s_scanl[threadIdx.x] = lines;
     __syncthreads();
     // inclusive Hillis-Steele scans over the block
+    // (the key-space sums saturate: a total beyond 2^63 fails the call below)
     for (int o = 1; o < (int)blockDim.x; o <<= 1) {
-      long long v = threadIdx.x >= (unsigned)o ? s_scan[threadIdx.x - o] : 0;
+      unsigned long long v = threadIdx.x >= (unsigned)o ? s_scan[threadIdx.x - o] : 0;
       long long vl = threadIdx.x >= (unsigned)o ? s_scanl[threadIdx.x - o] : 0;
       __syncthreads();
-      s_scan[threadIdx.x] += v;
+      s_scan[threadIdx.x] = sat_add(s_scan[threadIdx.x], v);
       s_scanl[threadIdx.x] += vl;
       __syncthreads();
     }
     if (i < d.ninst) {
-      d.inst[i].key_base = s_carry + s_scan[threadIdx.x] - cells;
+      d.inst[i].key_base = (int64_t)(sat_add(s_carry, s_scan[threadIdx.x]) - cells);
       d.inst[i].line_base = s_carryl + s_scanl[threadIdx.x] - lines;
     }
     __syncthreads();
     if (threadIdx.x == blockDim.x - 1) {
-      s_carry += s_scan[threadIdx.x];
+      s_carry = sat_add(s_carry, s_scan[threadIdx.x]);
       s_carryl += s_scanl[threadIdx.x];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    long long total = s_carry;
-    int bits = 1;
-    while (bits < 63 && (1LL << bits) < total) ++bits;
-    d.ctl[C_KEYBITS] = bits;
+    const unsigned long long total = s_carry;
     d.ctl[C_ERR] = s_err;
     d.ctl[C_NLINES] = s_carryl;
+    if (total > kMaxKeySpace) {
+      // the keys would not fit the sort: the whole call fails (instances of a
+      // batch share one key space), and every later kernel returns at once
+      d.ctl[C_KEYOVF] = 1;
+      d.ctl[C_ERR] = 1;
+      d.ctl[C_ABANDON] = 1;
+      d.ctl[C_KEYBITS] = 8;
+    } else {
+      d.ctl[C_KEYBITS] = key_bits_for(total);
+    }
   }
 }
 

@@ -139,9 +139,27 @@ enum Ctl : int {
                 // count: the kernels after it return at once
   C_IDLE,       // nothing pending for the current expansion pass
   C_ROUNDS_MAX, // most pop rounds of one expansion wave in one pass
+  C_KEYOVF,     // the cell-key space of the whole call exceeds 2^63 keys
   C_NCTL = 48
 };
-static_assert(C_ROUNDS_MAX < C_NCTL, "control block too small");
+static_assert(C_KEYOVF < C_NCTL, "control block too small");
+
+// Cell-key space: the sort key of a cell is its instance's key base plus its
+// linear index, and the radix sort orders keys of up to 63 bits, so the
+// cells of all instances of a call must number at most 2^63.  Sums are
+// saturating, so an overflowing total stays above the limit.
+constexpr unsigned long long kMaxKeySpace = 1ull << 63;
+__host__ __device__ __forceinline__ unsigned long long sat_add(unsigned long long a,
+                                                                unsigned long long b) {
+  const unsigned long long s = a + b;
+  return s < a ? ~0ull : s;
+}
+// Radix-sort key width for a key space of `total` keys (total <= 2^63).
+__host__ __device__ __forceinline__ int key_bits_for(unsigned long long total) {
+  int bits = 1;
+  while (bits < 63 && (1ull << bits) < total) ++bits;
+  return bits;
+}
 
 // Statistics counters are striped: a wave adds to one of kStripes copies.
 constexpr int kStripes = 64;

@@ -55,12 +55,17 @@ extern "C" {
  *                   (main.cu:192-207).  0 = cell width eps (auto).
  *   seed_list_size  MAX_SEEDS: seed-list entries per chain (0 = auto).
  *   blocks          THREAD_BLOCKS: workgroups of the expansion kernel
- *                   (0 = fill the device at resident occupancy).
- *   block_threads   THREAD_COUNT: threads per workgroup, multiple of 64
- *                   (0 = 256).
+ *                   (0 = fill the device at resident occupancy; rounded up
+ *                   to a multiple of 8, one share per XCD).
+ *   block_threads   THREAD_COUNT: threads per workgroup, 0..1024 (0 = 256).
+ *                   Each wavefront of 64 lanes owns one seed list, so the
+ *                   expansion runs the value rounded up to whole wavefronts,
+ *                   at most 256 threads (common.h:14's 16 runs as 64).
  *   extra_collision EXTRA_COLLISION_SIZE: collision-pair buffer entries per
  *                   workgroup (0 = auto).  Pairs beyond it are merged inline.
  *   precision       DCLUST_PRECISION_*.
+ * The launch values never change the labels; the shape a run actually used
+ * is reported in dclust_stats.eff_*.
  */
 typedef struct dclust_params {
   int32_t dim;
@@ -120,14 +125,22 @@ typedef struct dclust_stats {
    * the run was a replay of a captured graph, [15] most pop rounds (up to
    * 64 cells each) of one expansion wavefront in a pass */
   int64_t diag[16];
+  /* launch shape the run used, in common.h's terms: THREAD_BLOCKS,
+   * THREAD_COUNT (whole wavefronts), MAX_SEEDS, EXTRA_COLLISION_SIZE */
+  int32_t eff_blocks;
+  int32_t eff_block_threads;
+  int32_t eff_seed_list_size;
+  int32_t eff_extra_collision;
 } dclust_stats;
 
 typedef struct dclust_handle dclust_handle;
 
-/* common.h defaults: dim 2, eps 1.5, minpts 3, r 3, 128 seeds, 8 blocks of
- * 16 threads, 256 extra collisions -- except that blocks/threads/seeds/
- * extra_collision are set to 0 (auto) because the reference's 8x16 launch
- * is a debugging size. */
+/* common.h:13-25 defaults: DIMENSION 2, EPS 1.5, MINPTS 3, PARTITION_SIZE 3,
+ * MAX_SEEDS 128, THREAD_BLOCKS 8, THREAD_COUNT 16, EXTRA_COLLISION_SIZE 256,
+ * precision FP64.  The reference's 8 x 16 launch runs as 8 workgroups of
+ * one wavefront each (exact, but far below what the device holds); set
+ * blocks / block_threads / seed_list_size / extra_collision to 0 for the
+ * library's tuned shape. */
 int dclust_params_default(dclust_params* p);
 
 /* Opens a handle on HIP device `device`.  Replaces the device set-up and

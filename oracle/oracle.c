@@ -87,6 +87,38 @@ static int kv_cmp(const void* a, const void* b) {
   return (x->id > y->id) - (x->id < y->id);
 }
 
+/* Order (key, id) pairs by key, then id.  The pairs arrive in id order, so
+ * a stable LSD radix sort on the key (11-bit digits, as many as the key
+ * space needs) gives that order; qsort with kv_cmp when memory is short. */
+static int kv_sort(kv_t* kv, int64_t n, const grid_t* g) {
+  int64_t cells = 1;
+  for (int k = 0; k < g->dim; ++k) cells *= g->nk[k];
+  int bits = 1;
+  while (bits < 63 && ((int64_t)1 << bits) < cells) ++bits;
+  kv_t* tmp = (kv_t*)malloc(sizeof(kv_t) * (size_t)(n > 0 ? n : 1));
+  if (!tmp) {
+    qsort(kv, (size_t)n, sizeof(kv_t), kv_cmp);
+    return 0;
+  }
+  enum { RB = 11, RN = 1 << RB };
+  int64_t* cnt = (int64_t*)malloc(sizeof(int64_t) * RN);
+  if (!cnt) { free(tmp); return -1; }
+  kv_t* src = kv;
+  kv_t* dst = tmp;
+  for (int sh = 0; sh < bits; sh += RB) {
+    memset(cnt, 0, sizeof(int64_t) * RN);
+    for (int64_t i = 0; i < n; ++i) cnt[(src[i].key >> sh) & (RN - 1)]++;
+    int64_t run = 0;
+    for (int d = 0; d < RN; ++d) { int64_t c = cnt[d]; cnt[d] = run; run += c; }
+    for (int64_t i = 0; i < n; ++i) dst[cnt[(src[i].key >> sh) & (RN - 1)]++] = src[i];
+    kv_t* t = src; src = dst; dst = t;
+  }
+  if (src != kv) memcpy(kv, src, sizeof(kv_t) * (size_t)n);
+  free(cnt);
+  free(tmp);
+  return 0;
+}
+
 static int grid_build(grid_t* g, const double* xyz, int64_t n, int dim,
                       double eps) {
   memset(g, 0, sizeof(*g));
@@ -117,7 +149,7 @@ static int grid_build(grid_t* g, const double* xyz, int64_t n, int dim,
     kv[i].key = gkey(g, c);
     kv[i].id = (int32_t)i;
   }
-  qsort(kv, (size_t)n, sizeof(kv_t), kv_cmp);
+  if (kv_sort(kv, n, g)) { free(kv); return -1; }
   g->idx = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   g->key = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
   g->start = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
@@ -400,6 +432,242 @@ int oracle_fast_dbscan(const double* xyz, int64_t n, int dim, double eps,
   free(core);
   free(par);
   free(cid);
+  grid_free(&g);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The same labels for dense inputs, where the union step above visits       */
+/* thousands of neighbours per core point.  Core-core unions run over clique */
+/* sub-cells of side wc = 0.99 eps / sqrt(dim): any two points of one        */
+/* sub-cell differ by less than wc per axis, so d^2 < 0.9801 eps^2 (rounding */
+/* included) and every box test passes -- the core points of a sub-cell are */
+/* one set.  Two sub-cells at most ceil(eps / wc) = 2 apart per axis are     */
+/* united when some core pair across them satisfies P from either side      */
+/* (the union rule above); the scan of a pair stops at the first joined     */
+/* pair and is skipped when the two are already one set.  Near pairs (one   */
+/* cell apart) go first, so that far pairs inside a cluster are skipped.    */
+/* Core flags and border labels are computed as above.                      */
+/* ------------------------------------------------------------------------ */
+static void key_coords(const grid_t* g, int64_t key, int64_t* c) {
+  for (int k = g->dim - 1; k >= 0; --k) {
+    c[k] = key % g->nk[k];
+    key /= g->nk[k];
+  }
+}
+
+/* Unite sub-cell a with every sub-cell b > a in its 5^D stencil whose
+ * offset is near (max |d| <= 1) or far (max |d| == 2), per `far`. */
+static void cells_pass(const grid_t* s, const double* xyz, int dim, double eps,
+                       const uint8_t* core, const int32_t* rep, int32_t* par,
+                       int far) {
+  const double eps2 = eps * eps;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t a = 0; a < s->ncell; ++a) {
+    if (rep[a] < 0) continue;
+    int64_t ca[3] = {0, 0, 0}, cb[3] = {0, 0, 0};
+    key_coords(s, s->key[a], ca);
+    const int last = dim - 1;
+    for (int d0 = -2; d0 <= 2; ++d0)
+      for (int d1 = (dim == 3 ? -2 : 0); d1 <= (dim == 3 ? 2 : 0); ++d1) {
+        cb[0] = ca[0] + d0;
+        if (dim == 3) cb[1] = ca[1] + d1;
+        if (cb[0] < 0 || cb[0] >= s->nk[0]) continue;
+        if (dim == 3 && (cb[1] < 0 || cb[1] >= s->nk[1])) continue;
+        int64_t zlo = ca[last] - 2 < 0 ? 0 : ca[last] - 2;
+        int64_t zhi = ca[last] + 2 > s->nk[last] - 1 ? s->nk[last] - 1 : ca[last] + 2;
+        cb[last] = zlo;
+        int64_t klo = gkey(s, cb);
+        cb[last] = zhi;
+        int64_t khi = gkey(s, cb);
+        for (int64_t b = lower_bound64(s->key, s->ncell, klo);
+             b < s->ncell && s->key[b] <= khi; ++b) {
+          if (b <= a || rep[b] < 0) continue;
+          int64_t cc[3] = {0, 0, 0};
+          key_coords(s, s->key[b], cc);
+          int m = 0;
+          for (int k = 0; k < dim; ++k) {
+            int64_t d = cc[k] - ca[k];
+            if (d < 0) d = -d;
+            if (d > m) m = (int)d;
+          }
+          if ((m == 2) != (far != 0)) continue;
+          if (uf_find(par, rep[a]) == uf_find(par, rep[b])) continue;
+          int joined = 0;
+          for (int64_t i = s->start[a]; i < s->start[a + 1] && !joined; ++i) {
+            const int32_t p = s->idx[i];
+            if (!core[p]) continue;
+            const double* pp = xyz + (int64_t)p * dim;
+            for (int64_t j = s->start[b]; j < s->start[b + 1]; ++j) {
+              const int32_t q = s->idx[j];
+              if (!core[q]) continue;
+              const double* qp = xyz + (int64_t)q * dim;
+              if (pred_inline(pp, qp, dim, eps, eps2) || pred_inline(qp, pp, dim, eps, eps2)) {
+                joined = 1;
+                break;
+              }
+            }
+          }
+          if (joined) uf_union(par, rep[a], rep[b]);
+        }
+      }
+  }
+}
+
+int oracle_cells_dbscan(const double* xyz, int64_t n, int dim, double eps,
+                        int minpts, int nthreads, int32_t* labels,
+                        uint8_t* core_out) {
+  if (dim != 2 && dim != 3) return -1;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  const double wc = 0.99 * eps / sqrt((double)dim);
+  grid_t g, s;
+  if (grid_build(&g, xyz, n, dim, eps)) return -2;
+  if (grid_build(&s, xyz, n, dim, wc)) { grid_free(&g); return -2; }
+  /* grid_build widens cells of huge extents; the sub-cells must stay cliques */
+  for (int k = 0; k < dim; ++k)
+    if (s.inv[k] != 1.0 / wc) { grid_free(&g); grid_free(&s); return -4; }
+  size_t nn = (size_t)(n > 0 ? n : 1);
+  uint8_t* core = (uint8_t*)calloc(nn, 1);
+  int32_t* par = (int32_t*)malloc(sizeof(int32_t) * nn);
+  int32_t* cid = (int32_t*)calloc(nn, sizeof(int32_t));
+  int32_t* rep = (int32_t*)malloc(sizeof(int32_t) * (size_t)(s.ncell > 0 ? s.ncell : 1));
+  if (!core || !par || !cid || !rep) return -2;
+  double eps2 = eps * eps;
+
+  /* a point whose sub-cell holds at least minpts points is core: the
+   * sub-cell is a clique, so all of them are its neighbours */
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < s.ncell; ++c)
+    if (s.start[c + 1] - s.start[c] >= minpts)
+      for (int64_t j = s.start[c]; j < s.start[c + 1]; ++j) core[s.idx[j]] = 1;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < n; ++i) {
+    if (core[i]) continue;
+    int64_t lo[3], hi[3];
+    const double* p = xyz + i * dim;
+    box_range(&g, p, eps, 0, lo, hi);
+    cnt_ctx c = {xyz, dim, eps, eps2, p, 0, minpts};
+    grid_visit(&g, lo, hi, cnt_visit, &c);
+    core[i] = c.cnt >= minpts;
+  }
+  for (int64_t i = 0; i < n; ++i) par[i] = (int32_t)i;
+  /* a sub-cell's core points: one set, rooted at its smallest id (the
+   * sub-cell lists ids in increasing order) */
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < s.ncell; ++c) {
+    int32_t r = -1;
+    for (int64_t j = s.start[c]; j < s.start[c + 1]; ++j) {
+      const int32_t q = s.idx[j];
+      if (!core[q]) continue;
+      if (r < 0) r = q; else par[q] = r;
+    }
+    rep[c] = r;
+  }
+  cells_pass(&s, xyz, dim, eps, core, rep, par, 0);
+  cells_pass(&s, xyz, dim, eps, core, rep, par, 1);
+
+  int32_t k = 0;
+  for (int64_t i = 0; i < n; ++i)
+    if (core[i] && uf_find(par, (int32_t)i) == i) cid[i] = ++k;
+  for (int64_t i = 0; i < n; ++i)
+    if (core[i]) cid[i] = cid[uf_find(par, (int32_t)i)];
+
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < n; ++i) {
+    if (core[i]) { labels[i] = cid[i]; continue; }
+    int64_t lo[3], hi[3];
+    const double* b = xyz + i * dim;
+    box_range(&g, b, eps, 1, lo, hi);
+    bor_ctx c = {xyz, dim, eps, eps2, b, core, cid, 0};
+    grid_visit(&g, lo, hi, bor_visit, &c);
+    labels[i] = c.best;
+  }
+  if (core_out) memcpy(core_out, core, (size_t)n);
+  free(core);
+  free(par);
+  free(cid);
+  free(rep);
+  grid_free(&g);
+  grid_free(&s);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sampled check of given labels against the definition, by enumerating    */
+/* each sampled point's eps-ball with the predicate (no labels of the       */
+/* oracle involved).  For sampled point s with core flag core[s]:           */
+/*   core s:     every core q with P(s,q) or P(q,s) has labels[q] == labels[s] */
+/*               (and labels[s] > 0);                                       */
+/*   non-core s: labels[s] == min{labels[q] : q core, P(q,s)}, 0 if none.   */
+/* out[0] core-label violations, out[1] border violations, out[2] predicate */
+/* evaluations, out[3] core points sampled.                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  const double* xyz; int dim; double eps, eps2; int32_t self; const double* p;
+  const uint8_t* core; const int32_t* lab; int64_t bad, evals; int32_t best;
+} smp_ctx;
+
+static int smp_core_visit(int32_t q, void* vctx) {
+  smp_ctx* c = (smp_ctx*)vctx;
+  c->evals++;
+  if (q == c->self || !c->core[q]) return 0;
+  const double* qp = c->xyz + (int64_t)q * c->dim;
+  if (pred_inline(c->p, qp, c->dim, c->eps, c->eps2) ||
+      pred_inline(qp, c->p, c->dim, c->eps, c->eps2))
+    if (c->lab[q] != c->lab[c->self]) c->bad++;
+  return 0;
+}
+
+static int smp_border_visit(int32_t q, void* vctx) {
+  smp_ctx* c = (smp_ctx*)vctx;
+  c->evals++;
+  if (!c->core[q]) return 0;
+  const double* qp = c->xyz + (int64_t)q * c->dim;
+  if (pred_inline(qp, c->p, c->dim, c->eps, c->eps2))
+    if (c->best == 0 || c->lab[q] < c->best) c->best = c->lab[q];
+  return 0;
+}
+
+int oracle_sample_check(const double* xyz, int64_t n, int dim, double eps,
+                        const int32_t* labels, const uint8_t* core,
+                        const int64_t* sample, int64_t m, int nthreads,
+                        int64_t* out) {
+  if (dim != 2 && dim != 3) return -1;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  grid_t g;
+  if (grid_build(&g, xyz, n, dim, eps)) return -2;
+  const double eps2 = eps * eps;
+  int64_t bad_core = 0, bad_border = 0, evals = 0, ncore = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : bad_core, bad_border, evals, ncore)
+  for (int64_t t = 0; t < m; ++t) {
+    const int64_t i = sample[t];
+    if (i < 0 || i >= n) { bad_core++; continue; }
+    int64_t lo[3], hi[3];
+    const double* p = xyz + i * dim;
+    smp_ctx c = {xyz, dim, eps, eps2, (int32_t)i, p, core, labels, 0, 0, 0};
+    box_range(&g, p, eps, 1, lo, hi);
+    if (core[i]) {
+      ncore++;
+      grid_visit(&g, lo, hi, smp_core_visit, &c);
+      bad_core += c.bad + (labels[i] <= 0);
+    } else {
+      grid_visit(&g, lo, hi, smp_border_visit, &c);
+      bad_border += labels[i] != c.best;
+    }
+    evals += c.evals;
+  }
+  out[0] = bad_core;
+  out[1] = bad_border;
+  out[2] = evals;
+  out[3] = ncore;
   grid_free(&g);
   return 0;
 }

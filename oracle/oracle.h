@@ -41,6 +41,26 @@ int oracle_fast_dbscan(const double* xyz, int64_t n, int dim, double eps,
                        int minpts, int nthreads, int32_t* labels,
                        uint8_t* core_out, int64_t* asym_pairs_out);
 
+/* The same labels and core flags as oracle_fast_dbscan, with the core-core
+ * unions done between clique sub-cells (side 0.99 eps / sqrt(dim)), each
+ * pair scanned until its first joined pair: fast on dense inputs, where
+ * every core point has thousands of neighbours.  Returns -4 when the extent
+ * is too large for the sub-cell grid (more than 2^20 cells per axis). */
+int oracle_cells_dbscan(const double* xyz, int64_t n, int dim, double eps,
+                        int minpts, int nthreads, int32_t* labels,
+                        uint8_t* core_out);
+
+/* Checks given labels / core flags at m sampled points against the
+ * definition, enumerating each sampled point's eps-ball with the predicate:
+ * a core point shares its label with every core point it is joined to; a
+ * non-core point has the smallest label among the core points whose
+ * predicate reaches it (0 if none).  out[0] core-label violations, out[1]
+ * border violations, out[2] predicate evaluations, out[3] sampled cores. */
+int oracle_sample_check(const double* xyz, int64_t n, int dim, double eps,
+                        const int32_t* labels, const uint8_t* core,
+                        const int64_t* sample, int64_t m, int nthreads,
+                        int64_t* out);
+
 /* O(N^2) brute-force neighbour counts (self included), for pinning. */
 int oracle_brute_counts(const double* xyz, int64_t n, int dim, double eps,
                         int32_t* counts);

@@ -38,6 +38,10 @@ def lib():
         L.oracle_seq_dbscan.argtypes = [_P, C.c_int64, C.c_int, C.c_double, C.c_int, _P]
         L.oracle_fast_dbscan.argtypes = [_P, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_int,
                                          _P, _P, _P]
+        L.oracle_cells_dbscan.argtypes = [_P, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_int,
+                                          _P, _P]
+        L.oracle_sample_check.argtypes = [_P, C.c_int64, C.c_int, C.c_double, _P, _P, _P,
+                                          C.c_int64, C.c_int, _P]
         L.oracle_brute_counts.argtypes = [_P, C.c_int64, C.c_int, C.c_double, _P]
         L.oracle_multicore_dbscan.argtypes = [_P, C.c_int64, C.c_int, C.c_double, C.c_int,
                                               C.c_int, _P]
@@ -87,6 +91,34 @@ def fast_dbscan(xyz, eps, minpts, nthreads=0):
                                   lab.ctypes.data, core.ctypes.data, C.byref(asym))
     assert rc == 0, rc
     return lab, core, asym.value
+
+
+def cells_dbscan(xyz, eps, minpts, nthreads=0):
+    """(labels, core flags): the labels of fast_dbscan, with the core-core
+    unions done between clique sub-cells (dense inputs)."""
+    x = _a(xyz)
+    lab = np.empty(x.shape[0], dtype=np.int32)
+    core = np.empty(x.shape[0], dtype=np.uint8)
+    rc = lib().oracle_cells_dbscan(x.ctypes.data, x.shape[0], x.shape[1], eps, minpts, nthreads,
+                                   lab.ctypes.data, core.ctypes.data)
+    assert rc == 0, rc
+    return lab, core
+
+
+def sample_check(xyz, eps, labels, core, sample, nthreads=0):
+    """Violations of the DBSCAN definition at the sampled points:
+    (core-label violations, border violations, predicate evaluations,
+    sampled core points)."""
+    x = _a(xyz)
+    lab = np.ascontiguousarray(labels, dtype=np.int32)
+    cf = np.ascontiguousarray(core, dtype=np.uint8)
+    smp = np.ascontiguousarray(sample, dtype=np.int64)
+    out = np.zeros(4, dtype=np.int64)
+    rc = lib().oracle_sample_check(x.ctypes.data, x.shape[0], x.shape[1], eps, lab.ctypes.data,
+                                   cf.ctypes.data, smp.ctypes.data, smp.shape[0], nthreads,
+                                   out.ctypes.data)
+    assert rc == 0, rc
+    return tuple(int(v) for v in out)
 
 
 def brute_counts(xyz, eps):

@@ -38,7 +38,9 @@ def test_exports_every_declared_symbol(header):
 
 def test_params_default_mirror_common_h():
     p = dclust.default_params()
-    # common.h:19-25 defaults (DIMENSION 2, MINPTS 3, EPS 1.5, PARTITION_SIZE 3)
+    # common.h:13-25: THREAD_BLOCKS 8, THREAD_COUNT 16, MAX_SEEDS 128,
+    # EXTRA_COLLISION_SIZE 256, DIMENSION 2, MINPTS 3, EPS 1.5, PARTITION_SIZE 3
+    assert (p.blocks, p.block_threads, p.seed_list_size, p.extra_collision) == (8, 16, 128, 256)
     assert (p.dim, p.minpts, p.eps, p.partitions_r) == (2, 3, 1.5, 3)
     assert p.precision == dclust.FP64
 
@@ -49,10 +51,28 @@ def test_strerror_and_version():
     assert "gfx950" in dclust.version()
 
 
-def test_struct_layout_matches_header():
-    assert ctypes.sizeof(dclust.Params) == 40
-    assert dclust.Params.eps.offset == 8
-    assert ctypes.sizeof(dclust.Stats) == 9 * 8 + 17 * 8 + 2 * 4 + 16 * 8
+def test_struct_layout_matches_header(tmp_path):
+    # the C compiler's own layout of include/dclust.h against the ctypes mirror
+    fields = {"dclust_params": dclust.Params, "dclust_stats": dclust.Stats}
+    src = ["#include <stddef.h>", "#include <stdio.h>", '#include "dclust.h"', "int main(void) {"]
+    for st, cls in fields.items():
+        src.append(f'  printf("{st} sizeof %zu\\n", sizeof({st}));')
+        for name, _ in cls._fields_:
+            src.append(f'  printf("{st} {name} %zu\\n", offsetof({st}, {name}));')
+    src += ["  return 0;", "}"]
+    (tmp_path / "layout.c").write_text("\n".join(src) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(tmp_path / "layout.c"), "-o",
+                    str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                               check=True).stdout.splitlines():
+        st, name, val = line.split()
+        got[(st, name)] = int(val)
+    for st, cls in fields.items():
+        assert got[(st, "sizeof")] == ctypes.sizeof(cls), st
+        for name, _ in cls._fields_:
+            assert got[(st, name)] == getattr(cls, name).offset, (st, name)
 
 
 def test_device_entry_points_reject_bad_arguments_before_any_gpu_call():

@@ -4,7 +4,11 @@
   (oracle/_ref) and the committed golden labels it produced;
 - the fast formulation against the faithful sequential restatement (2-D and
   3-D) -- the two share only the predicate;
-- core flags against an O(N^2) brute-force count;
+- core flags against an O(N^2) brute-force count, labels against the
+  definition evaluated by brute force (2-D and 3-D);
+- the clique-sub-cell formulation (cells_dbscan, the checker for dense
+  full-size inputs) against the fast one, the golden labels and brute force;
+  the sampled definition check against exact labels and planted errors;
 - the predicate against hand-built boundary pairs (closed box, exact eps^2
   ties, FMA-sensitive sums) whose answers are fixed by IEEE arithmetic;
 - the cluster-check restatement against the reference checker compiled in
@@ -95,6 +99,121 @@ def test_core_flags_vs_brute_force(dim):
     # a core point has a label; a point with count 1 (only itself) is noise
     lab = oracle.seq_dbscan(x, eps, mp)
     assert np.all(lab[cnt >= mp] > 0)
+
+
+def _brute_labels(x, eps, mp):
+    """Labels by the definition: components of the core graph (P from either
+    side joins two core points), numbered in order of their smallest core
+    index; a border point takes the smallest id of a core point whose
+    predicate reaches it; noise 0.  O(N^2) predicate calls."""
+    n = x.shape[0]
+    adj = np.zeros((n, n), dtype=bool)
+    for i in range(n):
+        for j in range(n):
+            adj[i, j] = oracle.pred(x[i], x[j], eps)
+    core = adj.sum(axis=1) >= mp
+    comp = np.zeros(n, dtype=np.int32)
+    k = 0
+    for s in range(n):
+        if core[s] and comp[s] == 0:
+            k += 1
+            stack = [s]
+            comp[s] = k
+            while stack:
+                u = stack.pop()
+                for v in np.nonzero((adj[u] | adj[:, u]) & core)[0]:
+                    if comp[v] == 0:
+                        comp[v] = k
+                        stack.append(v)
+    lab = comp.copy()
+    for b in np.nonzero(~core)[0]:
+        ids = [comp[q] for q in np.nonzero(core & adj[:, b])[0]]
+        lab[b] = min(ids) if ids else 0
+    return lab, core.astype(np.uint8)
+
+
+@pytest.mark.parametrize("dim,eps", [(2, 0.15), (3, 0.4)])
+def test_labels_equal_brute_force_definition(dim, eps):
+    rng = np.random.default_rng(60 + dim)
+    x = rng.normal(0, 1, (700, dim))
+    x[:40] = x[40:80] + rng.normal(0, 0.3 * eps, (40, dim))  # some dense spots
+    mp = 4
+    ref, rcore = _brute_labels(x, eps, mp)
+    assert 3 < ref.max() and (ref == 0).sum() > 20 and (rcore == 0).sum() > (ref == 0).sum()
+    lab, core, _ = oracle.fast_dbscan(x, eps, mp)
+    assert np.array_equal(core, rcore) and np.array_equal(lab, ref)
+    lab, core = oracle.cells_dbscan(x, eps, mp)
+    assert np.array_equal(core, rcore) and np.array_equal(lab, ref)
+    assert np.array_equal(oracle.seq_dbscan(x, eps, mp), ref)
+
+
+def _dense_blobs(dim, n, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0, 1, (6, dim))
+    x = c[rng.integers(0, 6, n)] + rng.normal(0, 0.004, (n, dim))
+    k = n // 50
+    x[:k] = rng.uniform(0, 1, (k, dim))
+    t = rng.uniform(0, 1, (n // 20, 1))
+    x[k:k + n // 20] = c[0] + t * (c[1] - c[0]) + rng.normal(0, 2e-4, (n // 20, dim))
+    return x
+
+
+@pytest.mark.parametrize("case", ["blobs2", "blobs3", "dense2", "dense3", "dense3_mp60",
+                                  "cfg3", "cfg5", "cfg2"])
+def test_cells_matches_fast(case):
+    if case.startswith("blobs"):
+        dim = int(case[-1])
+        rng = np.random.default_rng(70 + dim)
+        x = rng.uniform(0, 1, (6, dim))[rng.integers(0, 6, 30000)] + rng.normal(0, 0.03,
+                                                                                (30000, dim))
+        eps, mp = (0.01, 5) if dim == 2 else (0.02, 5)
+    elif case.startswith("dense"):
+        dim = int(case[5])
+        x = _dense_blobs(dim, 60000, 80 + dim)
+        eps, mp = 0.002, (60 if case.endswith("mp60") else 20)
+    else:
+        x, info = dclust.synth_config(int(case[-1]), n=150000)
+        eps, mp = info["eps"], info["minpts"]
+    ref, rcore, asym = oracle.fast_dbscan(x, eps, mp)
+    assert asym == 0 and ref.max() > 1
+    lab, core = oracle.cells_dbscan(x, eps, mp)
+    assert np.array_equal(core, rcore)
+    assert np.array_equal(lab, ref)
+
+
+def test_cells_golden(golden):
+    x, info = dclust.synth_config(1)
+    lab, _ = oracle.cells_dbscan(x, info["eps"], info["minpts"])
+    assert np.array_equal(lab, golden["cfg1_labels"].astype(np.int32))
+    for k in range(3):
+        eps, mp = golden[f"lat{k}_par"]
+        lab, _ = oracle.cells_dbscan(golden[f"lat{k}_x"], float(eps), int(mp))
+        assert np.array_equal(lab, golden[f"lat{k}_labels"].astype(np.int32))
+    assert oracle.cells_dbscan(EX, 1.5, 3)[0].tolist() == [1, 1, 1, 2, 2, 2, 3, 3, 3]
+
+
+def test_sample_check_accepts_exact_labels_and_finds_planted_errors():
+    x = _dense_blobs(3, 40000, 91)
+    eps, mp = 0.002, 20
+    lab, core = oracle.cells_dbscan(x, eps, mp)
+    every = np.arange(x.shape[0])
+    bc, bb, ev, nc = oracle.sample_check(x, eps, lab, core, every)
+    assert (bc, bb) == (0, 0) and nc == int(core.sum()) and ev > x.shape[0]
+    # a core point moved to another cluster
+    ci = int(np.nonzero(core)[0][0])
+    bad = lab.copy()
+    bad[ci] = lab.max() + 1
+    assert oracle.sample_check(x, eps, bad, core, every)[0] > 0
+    # a border point given a larger adjacent id / noise given a cluster
+    nb = np.nonzero((core == 0) & (lab > 0))[0]
+    assert nb.size > 0
+    bad = lab.copy()
+    bad[nb[0]] = lab[nb[0]] + 1
+    assert oracle.sample_check(x, eps, bad, core, every)[1] == 1
+    nz = np.nonzero(lab == 0)[0]
+    bad = lab.copy()
+    bad[nz[:3]] = 1
+    assert oracle.sample_check(x, eps, bad, core, every)[1] == 3
 
 
 def test_labels_are_components_brute_force():
