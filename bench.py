@@ -93,28 +93,69 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def b_nbr(st, dim, bytes_per_coord):
-    """Algorithmic bytes of one neighbour-count launch: every point read once
-    as a query and once per neighbouring cell tile (sum over cells of n_c
-    times the points of its 5^D stencil of half-eps cells), 8 bytes per
-    stencil-cell range entry, 1 byte of core flag written per point, plus
-    2*D*8 bytes of FP64 coordinates per guard-band recheck.  The sums come
-    from one untimed run with DCLUST_NBR_STATS=1."""
+def stencil_bytes(st, dim, bytes_per_coord):
+    """Bytes a scan of every point's whole 5^D stencil of half-eps cells
+    would read: every point once as a query and once per neighbouring cell
+    tile (sum over cells of n_c times the points of its stencil), 8 bytes per
+    stencil-cell range entry, 1 byte of core flag per point, plus 2*D*8 bytes
+    of FP64 coordinates per guard-band recheck.  The neighbour-count kernel
+    stops at minpts and skips all-core cells, so it reads far less: this is a
+    size of the search problem, not the kernel's traffic.  The sums come from
+    one untimed run with DCLUST_NBR_STATS=1."""
     n = st["n_points"]
     return (bytes_per_coord * dim * (n + st["cand_total"]) + 8 * st["nbr_cells_total"] + n +
             16 * dim * st.get("band_rechecks_core", 0))
 
 
+def kernel_bytes(st, dim, f32):
+    """Algorithmic bytes of one launch of each neighbour kernel: the work the
+    kernel reported for this run (stats.work, pairs_evaluated, diag) times the
+    bytes one unit of that work reads or writes, with no credit for caches.
+
+    k_core   per point: cell_of 4, its cell's two starts 8, its coordinates
+             s*D, core flag 1; per line group 5 packed rows (20); per
+             non-empty line its point range (two starts, 8); per candidate
+             its coordinates s*D; per cell the core count 4 and mask clear 8;
+             per listed core point of a light cell the list entry 4.
+    k_adj    per lane (core point of a light cell): list entry 4, cell_of 4,
+             cell_z 4, fixed-point coordinates 4*D (+ FP64 8*D in the FP64
+             form), 3^(D-1) packed near rows; per non-empty near line its first
+             start 4; per cell on it: next start, cell_z, core count (12); per
+             pair test: core flag 1 + coordinates s*D; per core cell the mask 8.
+    k_rows   per cell and stencil line: line-table entry 8, packed row
+             written 4; per cell and group of 5 lines: cell_xyz 8 (the
+             last-coordinate loads of its searches are not counted: a lower
+             bound).
+    s = 4 (fixed point, FP32 form) or 8 (FP64 form)."""
+    n, c = st["n_points"], st["n_cells"]
+    s = 4 if f32 else 8
+    w = st["work"]
+    lines_all = 25 if dim == 3 else 5
+    near = 9 if dim == 3 else 3
+    groups = lines_all // 5
+    a = st["n_adj_points"]
+    core = (n * (4 + 8 + s * dim + 1) + 20 * w[0] + 8 * w[1] + s * dim * st["pairs_evaluated"] +
+            c * (4 + 8) + 4 * a)
+    lane = 12 + 4 * dim + (0 if f32 else 8 * dim) + 4 * near
+    adj = a * lane + 4 * w[2] + 12 * w[3] + st["diag"][8] * (1 + s * dim) + 8 * st["n_core_cells"]
+    rows = c * (lines_all * (8 + 4) + groups * 8)
+    return {"k_core": core, "k_adj": adj, "k_rows": rows}
+
+
+KERNEL_TIME = {"k_core": "t_core_ms", "k_adj": "t_adj_ms", "k_rows": "t_rows_ms"}
+
+
 def measured_traffic(workload, precision):
-    """HBM bytes per launch of k_core on this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated by a plain copy),
-    or None when no such measurement is committed."""
-    path = os.path.join(ROOT, "profiles", f"k_core_{workload}_{precision}_traffic.json")
+    """HBM bytes per launch of each profiled kernel on this workload (rocprofv3
+    FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by what the counter
+    reads for a plain device-to-device copy of known size), from the
+    committed summary, or {} when none is committed."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{precision}.json")
     try:
         with open(path) as f:
-            return json.load(f)["traffic_bytes"]
-    except (OSError, KeyError, ValueError):
-        return None
+            return {k: v["traffic_bytes"] for k, v in json.load(f)["kernels"].items()}
+    except (OSError, KeyError, ValueError, TypeError):
+        return {}
 
 
 def spatial_sample(x, m):
@@ -333,6 +374,119 @@ def batched_leg(hip, h, steps, warmup, check_oracle=True, count=4096):
     return out
 
 
+def _short(name):
+    """Kernel name without namespace, template and arguments."""
+    n = name.split("(")[0].split(" ")[-1]
+    return n.split("::")[-1].split("<")[0]
+
+
+def summarize_profile(path, skip_runs, traffic=None):
+    """Markdown summary of rocprofv3 CSV output under `path` (one profiled
+    bench command per subdirectory or file prefix): per kernel, its time and
+    counters per clustering run, averaged over the runs after the first
+    `skip_runs` (a run starts at its k_init_head dispatch).  FETCH_SIZE and
+    WRITE_SIZE are divided by the factor the same counter shows for the
+    1 GiB device-to-device copy that --calib-copy 1024 makes before the runs."""
+    import csv
+    import glob
+
+    out = []
+    for f in sorted(glob.glob(os.path.join(path, "**", "*_kernel_trace.csv"), recursive=True)):
+        rows = sorted(csv.DictReader(open(f)), key=lambda r: int(r["Start_Timestamp"]))
+        runs, cur = [], None
+        for r in rows:
+            k = _short(r["Kernel_Name"])
+            if k == "k_init_head":
+                cur = {}
+                runs.append(cur)
+            if cur is None:
+                continue
+            d = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+            t, c = cur.get(k, (0, 0))
+            cur[k] = (t + d, c + 1)
+        timed = runs[skip_runs:]
+        tot = {}
+        for run in timed:
+            for k, (t, c) in run.items():
+                a = tot.setdefault(k, [0, 0, []])
+                a[0] += t
+                a[1] += c
+                a[2].append(t)
+        out.append(f"### Kernel times per clustering run: `{os.path.relpath(f, path)}`\n")
+        out.append(f"{len(runs)} runs in the trace, the first {skip_runs} left out, "
+                   f"{len(timed)} averaged.\n")
+        out.append("| Kernel | dispatches / run | avg us / run | min us | max us | share |")
+        out.append("|---|---:|---:|---:|---:|---:|")
+        whole = sum(v[0] for v in tot.values())
+        for k, (t, c, per) in sorted(tot.items(), key=lambda kv: -kv[1][0]):
+            out.append(f"| `{k}` | {c / len(timed):.2f} | {t / len(timed) / 1e3:.1f} | "
+                       f"{min(per) / 1e3:.1f} | {max(per) / 1e3:.1f} | {100 * t / whole:.1f} % |")
+        out.append(f"\nSum of kernel time per run: {whole / len(timed) / 1e3:.1f} us\n")
+    for f in sorted(glob.glob(os.path.join(path, "**", "*_counter_collection.csv"),
+                              recursive=True)):
+        rows = list(csv.DictReader(open(f)))
+        disp = {}
+        for r in rows:
+            e = disp.setdefault(int(r["Dispatch_Id"]), {"k": _short(r["Kernel_Name"]), "c": {}})
+            e["c"][r["Counter_Name"]] = e["c"].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        order = sorted(disp)
+        calib, runs, cur = {}, [], None
+        for i in order:
+            e = disp[i]
+            if e["k"] == "k_init_head":
+                cur = {}
+                runs.append(cur)
+            if cur is None:
+                if "copyBuffer" in e["k"]:
+                    for cn, v in e["c"].items():
+                        calib[cn] = calib.get(cn, 0.0) + v
+                continue
+            a = cur.setdefault(e["k"], {})
+            for cn, v in e["c"].items():
+                a[cn] = a.get(cn, 0.0) + v
+        timed = runs[skip_runs:]
+        names = sorted({cn for run in timed for a in run.values() for cn in a})
+        factor = {cn: calib[cn] / (1 << 20) for cn in ("FETCH_SIZE", "WRITE_SIZE")
+                  if cn in calib and calib[cn] > 0}
+        out.append(f"### Counters per clustering run: `{os.path.relpath(f, path)}`\n")
+        out.append(f"{len(runs)} runs, the first {skip_runs} left out, {len(timed)} averaged.")
+        for cn, fa in factor.items():
+            out.append(f"{cn}: the 1 GiB copy reads {calib[cn]:.0f} KiB, factor {fa:.4f}; "
+                       f"MB below = KiB / factor * 1024 / 1e6.")
+        out.append("")
+        hdr = "| Kernel | " + " | ".join(names) + (" | HBM MB |" if factor else " |")
+        out.append(hdr)
+        out.append("|---|" + "---:|" * (len(names) + (1 if factor else 0)))
+        avg = {}
+        for run in timed:
+            for k, a in run.items():
+                b = avg.setdefault(k, {})
+                for cn, v in a.items():
+                    b[cn] = b.get(cn, 0.0) + v / len(timed)
+        # ratio metrics (percentages, occupancy, conflicts per access) are
+        # per dispatch: averaged over a run's dispatches instead of summed
+        ndisp = {}
+        for i in order:
+            e = disp[i]
+            ndisp[e["k"]] = ndisp.get(e["k"], 0) + 1
+        ratio = [cn for cn in names if not (cn.endswith("_SIZE") or cn.endswith("_sum"))]
+        for k, b in avg.items():
+            per_run = ndisp.get(k, 1) / max(1, len(runs))
+            for cn in ratio:
+                if cn in b:
+                    b[cn] /= max(1.0, per_run)
+        key = names[0] if names else None
+        for k, b in sorted(avg.items(), key=lambda kv: -kv[1].get(key, 0.0)):
+            cells = [f"{b.get(cn, 0.0):.4g}" for cn in names]
+            mb = sum(b.get(cn, 0.0) / fa * 1024 / 1e6 for cn, fa in factor.items())
+            out.append(f"| `{k}` | " + " | ".join(cells) + (f" | {mb:.1f} |" if factor else " |"))
+            if traffic is not None:
+                for cn, fa in factor.items():
+                    traffic.setdefault(k, {})[cn] = b.get(cn, 0.0) / fa * 1024
+        out.append("")
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -347,10 +501,38 @@ def main():
                     help="skip the batched-vs-per-instance leg (cfg4)")
     ap.add_argument("--no-onestep", action="store_true",
                     help="skip the one-step (CUDA-DClust design) baseline leg")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="the timed leg only (no neighbourhood-size run, no FP64 comparison, "
+                         "parity, batched, one-step or CPU legs): the command profiled "
+                         "under rocprofv3")
+    ap.add_argument("--summarize-profile", metavar="DIR",
+                    help="print a markdown summary of the rocprofv3 CSV output under DIR "
+                         "(per clustering run, warm-up runs left out) and exit")
+    ap.add_argument("--traffic-out", metavar="FILE",
+                    help="with --summarize-profile: write the per-kernel HBM bytes "
+                         "(FETCH_SIZE + WRITE_SIZE, calibrated) as JSON")
+    ap.add_argument("--skip-runs", type=int, default=1,
+                    help="runs left out of --summarize-profile (the eager warm-up)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a plain copy of known size, "
                          "used to calibrate the HBM byte counters of a profiler run)")
     args = ap.parse_args()
+    if args.summarize_profile:
+        traffic = {} if args.traffic_out else None
+        print(summarize_profile(args.summarize_profile, args.skip_runs, traffic))
+        if args.traffic_out:
+            # HBM bytes per launch of the kernels whose FETCH_SIZE and WRITE_SIZE
+            # were both collected (the bench's roofline `traffic`)
+            ks = {k: {"traffic_bytes": int(v["FETCH_SIZE"] + v["WRITE_SIZE"]),
+                      "fetch_bytes": int(v["FETCH_SIZE"]), "write_bytes": int(v["WRITE_SIZE"])}
+                  for k, v in traffic.items() if "FETCH_SIZE" in v and "WRITE_SIZE" in v}
+            with open(args.traffic_out, "w") as f:
+                json.dump({"source": args.summarize_profile, "kernels": ks}, f, indent=1,
+                          sort_keys=True)
+        return
+    if args.profile_run:
+        args.no_cpu_baseline = args.no_parity = args.no_fp64_compare = True
+        args.no_batched = args.no_onestep = True
     ws = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or ws != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
@@ -388,26 +570,37 @@ def main():
         hip.sync()
         return time.perf_counter() - t0, sts
 
-    # neighbourhood sizes for the algorithmic-byte count: one extra untimed run
-    os.environ["DCLUST_NBR_STATS"] = "1"
     log(f"{args.workload}: {n} points generated")
-    st_nbr = h.run_device(d_x, n, p, d_lab)
-    del os.environ["DCLUST_NBR_STATS"]
+    st_nbr = None
+    if not args.profile_run:
+        # neighbourhood sizes for the stencil-byte figure: one extra untimed
+        # run (its k_rows makes extra loads, so it is kept out of profiles)
+        os.environ["DCLUST_NBR_STATS"] = "1"
+        st_nbr = h.run_device(d_x, n, p, d_lab)
+        del os.environ["DCLUST_NBR_STATS"]
     dt, sts = timed(p, args.steps, args.warmup)
     log(f"timed leg done: {dt / args.steps * 1e3:.3f} ms per clustering")
     st = dict(sts[-1])
-    st["cand_total"] = st_nbr["cand_total"]
-    st["nbr_cells_total"] = st_nbr["nbr_cells_total"]
     ms_step = dt / args.steps * 1e3
     value = n * args.steps / dt / 1e6
-    t_core = float(np.mean([s["t_core_ms"] for s in sts]))
     phases = {k: round(float(np.mean([s[k] for s in sts])), 4)
-              for k in ("t_index_ms", "t_core_ms", "t_adj_ms", "t_expand_ms", "t_merge_ms", "t_far_ms",
-                        "t_border_ms", "t_relabel_ms", "t_total_ms")}
-    bpc = 4 if prec == dclust.FP32_GUARD else 8
-    bytes_core = b_nbr(st, dim, bpc)
-    achieved = bytes_core / (t_core * 1e-3) / 1e9
+              for k in ("t_index_ms", "t_rows_ms", "t_core_ms", "t_adj_ms", "t_expand_ms",
+                        "t_merge_ms", "t_far_ms", "t_border_ms", "t_relabel_ms", "t_total_ms")}
+    f32 = prec == dclust.FP32_GUARD
+    kb = kernel_bytes(st, dim, f32)
     traffic = measured_traffic(args.workload, args.precision)
+    kernels = {}
+    for k, tk in KERNEL_TIME.items():
+        t_k = float(np.mean([s[tk] for s in sts]))  # live HIP events, library stream
+        kernels[k] = {"avg_launch_ms": round(t_k, 5), "algorithmic_bytes": int(kb[k]),
+                      "achieved": round(kb[k] / (t_k * 1e-3) / 1e9, 1),
+                      "frac": round(kb[k] / (t_k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "traffic": traffic.get(k),
+                      "hbm_frac": (round(traffic[k] / (t_k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                   if traffic.get(k) else None)}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_launch_ms"])
+    kd = kernels[dom]
+    t_core = kernels["k_core"]["avg_launch_ms"]
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mpoints/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
@@ -415,28 +608,31 @@ def main():
         "dtype": "f32" if prec == dclust.FP32_GUARD else "f64", "data": "synthetic",
         "config": {"workload": desc, "n_points": n, "dim": dim, "eps": info["eps"],
                    "minpts": info["minpts"],
-                   "precision": "fp32+fp64 guard band" if bpc == 4 else "fp64"},
-        "roofline": {"kernel": "k_core (neighbour count)", "bound": "hbm",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic,
-                     # the HBM bytes the profiler measured per launch over this launch time:
-                     # the stencil re-reads counted in `achieved` are served by the caches
-                     "hbm_achieved": round(traffic / (t_core * 1e-3) / 1e9, 1) if traffic else None,
-                     "hbm_frac": (round(traffic / (t_core * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-                                  if traffic else None),
-                     "algorithmic_bytes": int(bytes_core), "avg_launch_ms": round(t_core, 5),
-                     # pairs a scan of every point's 5^D half-eps stencil would test, and the
-                     # distance evaluations the kernel makes, per second of its launch
-                     "stencil_pairs_per_s": round(st["cand_total"] / (t_core * 1e-3), 1),
-                     "distance_evals_per_s": round(st["pairs_evaluated"] / (t_core * 1e-3), 1)},
+                   "precision": "fp32+fp64 guard band" if f32 else "fp64"},
+        # the longest single kernel of the run (live HIP events): its algorithmic
+        # bytes from the work it did, and the HBM bytes rocprofv3 measured
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": kd["achieved"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kd["frac"],
+                     "traffic": kd["traffic"], "hbm_frac": kd["hbm_frac"],
+                     "algorithmic_bytes": kd["algorithmic_bytes"],
+                     "avg_launch_ms": kd["avg_launch_ms"]},
+        "kernels": kernels,
+        # distance evaluations of the neighbour-count kernel per second of its launch
+        "distance_evals_per_s": round(st["pairs_evaluated"] / (t_core * 1e-3), 1),
         "phases_ms": phases,
         "counters": {k: st[k] for k in ("n_cells", "n_core", "n_noise", "n_clusters", "n_chains",
                                         "n_collisions", "n_overflow", "n_passes",
                                         "band_rechecks", "band_flips", "asym_pairs",
-                                        "pairs_evaluated", "cand_total", "key_bits",
-                                        "rec_overflow", "diag")},
+                                        "pairs_evaluated", "key_bits", "rec_overflow",
+                                        "n_core_cells", "n_adj_points", "n_lines", "work",
+                                        "diag")},
     }
+    if st_nbr is not None:
+        sb = stencil_bytes(dict(st, cand_total=st_nbr["cand_total"],
+                                nbr_cells_total=st_nbr["nbr_cells_total"]), dim, 4 if f32 else 8)
+        out["stencil_search"] = {"stencil_bytes": int(sb), "stencil_pairs": st_nbr["cand_total"],
+                                 "stencil_pairs_per_s": round(st_nbr["cand_total"] /
+                                                              (t_core * 1e-3), 1)}
     if not args.no_fp64_compare and prec == dclust.FP32_GUARD:
         p64 = dclust.make_params(dim, info["eps"], info["minpts"], r=info["partitions_r"],
                                  seeds=info["seed_list_size"], precision=dclust.FP64)

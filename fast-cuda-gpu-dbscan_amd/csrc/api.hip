@@ -36,8 +36,8 @@ struct Buf {
 };
 
 enum Ev {
-  E_START, E_INDEX, E_CORE0, E_CORE, E_ADJ, E_PASS0, E_EXPAND, E_MERGE, E_FAR, E_BORDER, E_END,
-  E_N
+  E_START, E_ROWS0, E_INDEX, E_CORE0, E_CORE, E_ADJ, E_PASS0, E_EXPAND, E_MERGE, E_FAR, E_BORDER,
+  E_END, E_N
 };
 
 }  // namespace
@@ -289,6 +289,10 @@ hipError_t capture_run(dclust_handle* h, const Dev& d, const RunShape& shape, bo
   if (body == hipSuccess) {
     enqueue_head(d, nchunks, s);
     launch_index_tail(d, d.n, nchunks, grid_cells, sort_passes, s);
+    body = mark(h, E_ROWS0, true);
+  }
+  if (body == hipSuccess) {
+    launch_rows(d, d.n, grid_cells, s);
     body = enqueue_core(h, d, shape, f32, true);
   }
   for (int p = 0; body == hipSuccess && p < passes; ++p) {
@@ -352,6 +356,14 @@ void finish_run(dclust_handle* h, const Dev& d, const RunShape& shape, long long
   h->last_core_out = d.core_out;
   if (!stats) return;
   stats->t_index_ms = ev_ms(h, E_START, E_CORE0);
+  stats->t_rows_ms = ev_ms(h, E_ROWS0, E_INDEX);
+  stats->n_core_cells = ctl_h[C_NCORECELLS];
+  stats->n_adj_points = ctl_h[C_NADJPTS];
+  stats->n_lines = ctl_h[C_NLINES];
+  stats->work[0] = ctl_h[C_CORE_GROUPS];
+  stats->work[1] = ctl_h[C_CORE_LINES];
+  stats->work[2] = ctl_h[C_ADJ_LINES];
+  stats->work[3] = ctl_h[C_ADJ_CELLS];
   stats->t_core_ms = ev_ms(h, E_CORE0, E_CORE);
   stats->t_adj_ms = ev_ms(h, E_CORE, E_ADJ);
   stats->t_expand_ms = ev_ms(h, E_ADJ, E_EXPAND);
@@ -641,6 +653,8 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
       d.line_cap = nlines;
     }
     launch_index_tail(d, ntot, nchunks, grid_cells, passes, s);
+    HIPCHK(mark(h, E_ROWS0, false));
+    launch_rows(d, ntot, grid_cells, s);
     if (onestep) {
       HIPCHK(mark(h, E_INDEX, false));
       return onestep_loop(h, d, params[0], onestep_threads, d_labels, stats);

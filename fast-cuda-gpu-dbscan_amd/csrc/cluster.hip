@@ -68,6 +68,7 @@ struct Batch {
 
 struct Tally {  // per-lane event counts (summed per wave by stat_add)
   int band = 0, flip = 0, asym = 0, eval = 0;
+  int groups = 0, lines = 0, cells = 0;  // stencil work (algorithmic bytes)
 };
 
 // FP32 squared distance from fixed-point coordinates (units of w*2^-29).
@@ -223,6 +224,9 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
   for (int g = 0; g < NR && cnt < P.minpts; g += 5) {
     int32_t aa[5], ll[5];
     line_group(d, rows, g, aa, ll);
+    ++t.groups;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) t.lines += ll[j] > 0;
     Stream5 s;
     s.set(aa, ll);
     for (int32_t t0 = 0; t0 < s.tot && cnt < P.minpts; t0 += B) {
@@ -339,6 +343,8 @@ __global__ __launch_bounds__(256) void k_core(Dev d) {
   stat_add(d, C_BAND_CORE, t.band);
   stat_add(d, C_BAND, t.band);
   stat_add(d, C_FLIP, t.flip);
+  stat_add(d, C_CORE_GROUPS, t.groups);
+  stat_add(d, C_CORE_LINES, t.lines);
 }
 
 // ------------------------------------------------------------- adjacency --
@@ -477,6 +483,8 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
     const int32_t pk = act ? rows[r] : 0;
     const int32_t len = pk >> kRowCntShift;
     if (len > 0) {
+      ++t.lines;
+      t.cells += len;
       const int32_t st = pk & kRowCellMask;
       int32_t bnd[6], nco[5], zb[5];
 #pragma unroll
@@ -603,6 +611,8 @@ __global__ __launch_bounds__(256, 5) void k_adj(Dev d) {
   stat_add(d, C_ASYM, t.asym);
   stat_add(d, C_ADJ_TESTS, t.eval);
   stat_add(d, C_ADJ_FREE, freeb);
+  stat_add(d, C_ADJ_LINES, t.lines);
+  stat_add(d, C_ADJ_CELLS, t.cells);
 }
 
 // Heavy cell A (more than kHeavy points), one wavefront, near cells only:

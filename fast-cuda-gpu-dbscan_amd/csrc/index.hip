@@ -872,8 +872,12 @@ void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
                      dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_line_clear, dim3(grid_cells), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_line_mark, dim3(grid_cells), dim3(256), 0, s, d);
+}
+
+void launch_rows(const Dev& d, int64_t n, int grid_cells, hipStream_t s) {
   // 4 x grid_cells workgroups per line group, but no more than the cells
   // (at most n) need: the dependent searches want many waves in flight
+  const int64_t gb = (n + 255) / 256;
   const int rb = (int)std::max<int64_t>(1, std::min<int64_t>(4LL * grid_cells, gb));
   hipLaunchKernelGGL(k_rows, dim3(rb, 5), dim3(256), 0, s, d);
 }

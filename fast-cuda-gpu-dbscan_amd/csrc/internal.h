@@ -140,9 +140,14 @@ enum Ctl : int {
   C_IDLE,       // nothing pending for the current expansion pass
   C_ROUNDS_MAX, // most pop rounds of one expansion wave in one pass
   C_KEYOVF,     // the cell-key space of the whole call exceeds 2^63 keys
+  // work counts of the neighbour kernels (their algorithmic bytes)
+  C_CORE_GROUPS,  // k_core: groups of 5 stencil lines streamed
+  C_CORE_LINES,   // k_core: non-empty stencil lines streamed
+  C_ADJ_LINES,    // k_adj: non-empty near lines examined
+  C_ADJ_CELLS,    // k_adj: cells of those lines
   C_NCTL = 48
 };
-static_assert(C_KEYOVF < C_NCTL, "control block too small");
+static_assert(C_ADJ_CELLS < C_NCTL, "control block too small");
 
 // Cell-key space: the sort key of a cell is its instance's key base plus its
 // linear index, and the radix sort orders keys of up to 63 bits, so the

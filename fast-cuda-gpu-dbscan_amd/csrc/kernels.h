@@ -11,9 +11,11 @@ void launch_init_head(const Dev& d, hipStream_t s);
 void launch_init_run(const Dev& d, int64_t waves, hipStream_t s);
 // bbox + grid set-up (per-instance cell width, cells per axis, key space)
 void launch_index_head(const Dev& d, int nchunks, hipStream_t s);
-// cell keys, `passes` radix passes, CSR compaction, gather, neighbour rows
+// cell keys, `passes` radix passes, CSR compaction, gather, line table
 void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
                        int passes, hipStream_t s);
+// neighbour rows of every cell (k_rows)
+void launch_rows(const Dev& d, int64_t n, int grid_cells, hipStream_t s);
 // exclusive in-place scan of nb int64 values by one block
 void launch_scan_top(int64_t* v, int nb, hipStream_t s);
 

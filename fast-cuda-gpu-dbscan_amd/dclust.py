@@ -59,16 +59,19 @@ _STAT_I64 = ["n_points", "n_cells", "n_core", "n_noise", "n_clusters", "n_chains
              "band_rechecks_core", "rec_overflow"]
 _STAT_I32 = ["hip_error", "key_bits"]
 _STAT_SHAPE = ["eff_blocks", "eff_block_threads", "eff_seed_list_size", "eff_extra_collision"]
+_STAT_COUNTS = ["n_core_cells", "n_adj_points", "n_lines"]
 
 
 class Stats(C.Structure):
     _fields_ = ([(k, C.c_double) for k in _STAT_F64] + [(k, C.c_int64) for k in _STAT_I64] +
                 [(k, C.c_int32) for k in _STAT_I32] + [("diag", C.c_int64 * 16)] +
-                [(k, C.c_int32) for k in _STAT_SHAPE])
+                [(k, C.c_int32) for k in _STAT_SHAPE] + [("t_rows_ms", C.c_double)] +
+                [(k, C.c_int64) for k in _STAT_COUNTS] + [("work", C.c_int64 * 4)])
 
     def as_dict(self):
-        out = {k: getattr(self, k) for k, _ in self._fields_ if k != "diag"}
+        out = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("diag", "work")}
         out["diag"] = list(self.diag)
+        out["work"] = list(self.work)
         return out
 
 

@@ -131,6 +131,14 @@ typedef struct dclust_stats {
   int32_t eff_block_threads;
   int32_t eff_seed_list_size;
   int32_t eff_extra_collision;
+  double t_rows_ms;       /* the neighbour-line kernel alone (inside t_index) */
+  int64_t n_core_cells;   /* cells holding core points                       */
+  int64_t n_adj_points;   /* core points of light cells (near-adjacency lanes) */
+  int64_t n_lines;        /* possible grid lines (line-table entries)        */
+  /* work of the neighbour kernels (the units of their algorithmic bytes):
+   * [0] k_core line groups streamed (5 lines each), [1] k_core non-empty
+   * lines, [2] k_adj non-empty near lines, [3] k_adj cells on them */
+  int64_t work[4];
 } dclust_stats;
 
 typedef struct dclust_handle dclust_handle;

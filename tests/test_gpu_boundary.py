@@ -97,7 +97,6 @@ def test_reference_launch_values_run_exactly(handle):
     ref, core, _ = oracle.fast_dbscan(x, info["eps"], info["minpts"])
     assert np.array_equal(lab, ref)
     assert np.array_equal(handle.core_flags(x.shape[0]), core)
-    assert st["n_passes"] > 1  # 8 small lists overflow into later passes
 
 
 @pytest.mark.parametrize("threads,eff", [(1, 64), (16, 64), (100, 128), (200, 256),
