@@ -93,9 +93,15 @@ def test_onestep_full_lists_and_shapes(handle, devmem, seeds, blocks, threads):
     _verify(x, 0.12, 6, lab, st)
 
 
-def test_onestep_bad_shape(handle, devmem):
-    with pytest.raises(dclust.DclustError):
-        _onestep(handle, devmem, EX, 1.5, 3, threads=100)
+def test_onestep_thread_counts(handle, devmem):
+    # THREAD_COUNT is rounded up to whole wavefronts (100 -> 128); beyond
+    # 1024 threads the call is refused
+    lab, st = _onestep(handle, devmem, EX, 1.5, 3, threads=100)
+    assert st["eff_block_threads"] == 128
+    assert lab.tolist() == [1, 1, 1, 2, 2, 2, 3, 3, 3]
+    with pytest.raises(dclust.DclustError) as e:
+        _onestep(handle, devmem, EX, 1.5, 3, threads=1025)
+    assert e.value.code == dclust.E_PARAM
 
 
 def test_onestep_contacts_folded_during_the_run(handle, devmem, monkeypatch):
