@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <vector>
 
@@ -66,6 +67,7 @@ struct dclust_handle {
   int gpasses = 0;                 // expansion passes it holds
   bool graph_broken = false;       // capture failed on this device: eager only
   bool graph_off = false;          // eager only (while autotuning)
+  uint8_t* scratch = nullptr;      // this device's scratch array (no-allocation runs)
 
   template <class T>
   T* get(int slot, size_t count, hipError_t* err) {
@@ -110,6 +112,62 @@ enum Slot {
 constexpr long long kMaxLineTable = 1LL << 27;  // 1 GiB at most
 // expansion passes a captured run may hold; runs that need more stay eager
 constexpr int kGraphMaxPasses = 8;
+
+// Device memory of dclust_run_device_noalloc: one array per GPU, placed with
+// the code object when it is loaded, so a run needs no allocation call.  In
+// host code `scratch` names a host-side stand-in, not device memory: its
+// device address is looked up once per device (scratch_address) and only
+// that address is handed to kernels.  Every no-allocation run on a GPU uses
+// the same array, so such runs on one GPU must not overlap.  The kernels
+// write every word they read before reading it (the run's own init kernels
+// clear counters, lists and forests), so nothing is cleared here.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch[kMaxScratchDevices] = {};
+
+// Device address of this device's scratch array (the device must be
+// current); looked up on the first call per device.
+hipError_t scratch_address(int device, uint8_t** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = (uint8_t*)p;
+  }
+  *out = g_scratch[device];
+  return hipSuccess;
+}
+
+// Where a run's device buffers come from: the handle's arena (grown with
+// hipMalloc and reused), or, for the no-allocation entry point, a fixed
+// region handed out by a bump pointer in 256-byte steps.  A region too small
+// for the run sets `over` and hands out nothing.
+struct RunMem {
+  dclust_handle* h;
+  uint8_t* base = nullptr;  // region; null = the arena
+  size_t cap = 0;
+  size_t used = 0;
+  bool over = false;
+  hipError_t err = hipSuccess;
+  template <class T>
+  T* get(int slot, size_t count) {
+    if (!base) return h->get<T>(slot, count, &err);
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    const size_t off = (used + 255) & ~(size_t)255;
+    if (off > cap || bytes > cap - off) {
+      over = true;
+      return nullptr;
+    }
+    used = off + bytes;
+    return (T*)(base + off);
+  }
+};
 
 int check_params(const dclust_params* p) {
   if (!p) return DCLUST_E_PARAM;
@@ -424,10 +482,14 @@ int onestep_loop(dclust_handle* h, const Dev& d, const dclust_params& prm, int t
 
 // onestep_threads > 0: the one-step baseline with workgroups of that size
 // runs after the index instead of the clustering phases
+// region != null: every device buffer of the run comes from that region
+// (cap bytes), nothing is allocated and the run is eager (no graph capture).
 int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64_t* n,
                const dclust_params* params, int32_t* d_labels, int32_t* inst_err,
-               dclust_stats* stats, int onestep_threads = 0) {
+               dclust_stats* stats, int onestep_threads = 0, uint8_t* region = nullptr,
+               size_t region_cap = 0) {
   const bool onestep = onestep_threads > 0;
+  const bool fixed_mem = region != nullptr;
   hipError_t herr = hipSuccess;
   int rc = DCLUST_OK;
   if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -470,6 +532,12 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
   if (ntot == 0) return DCLUST_OK;
 
   RunShape shape = pick_shape(h, params[0], ntot, f32);
+  if (fixed_mem && params[0].extra_collision == 0) {
+    // collision pairs beyond the per-wave buffer are merged inline: a
+    // smaller buffer leaves more of the fixed region to the points
+    shape.pair_per = 64;
+    shape.pair_cap = (int64_t)shape.pair_per * shape.waves;
+  }
   if (onestep) {  // the expansion buffers stay unused: keep them minimal
     shape.blocks = 8;
     shape.waves = 8 * (shape.threads / 64);
@@ -501,77 +569,84 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
   for (int32_t i = 0; i < ninst; ++i) rec_stride = std::max(rec_stride, params[i].minpts);
   rec_stride = std::min(rec_stride, kRecSlots);
   d.rec_stride = rec_stride;
+  RunMem mem{h, region, region_cap};
   {
-    hipError_t& E = herr;
-    d.inst = h->get<Inst>(S_INST, ninst, &E);
-    d.chunks = h->get<Chunk>(S_CHUNK, nchunks, &E);
-    d.bbox = h->get<unsigned long long>(S_BBOX, (size_t)ninst * kBoxStripes * 8, &E);
-    d.ctl = h->get<long long>(S_CTL, C_NCTL, &E);
-    d.stat = h->get<long long>(S_STAT, (size_t)C_NCTL * kStripes, &E);
-    d.key[0] = h->get<uint64_t>(S_KEY0, ntot, &E);
-    d.key[1] = h->get<uint64_t>(S_KEY1, ntot, &E);
-    d.val[0] = h->get<int32_t>(S_VAL0, ntot, &E);
-    d.val[1] = h->get<int32_t>(S_VAL1, ntot, &E);
-    d.hist = h->get<uint32_t>(S_HIST, sort_hist_entries(ntot), &E);
-    d.offs = h->get<uint32_t>(S_OFFS, sort_hist_entries(ntot), &E);
-    d.dtot = h->get<uint32_t>(S_DTOT, 256, &E);
-    d.pass_scatter = h->get<int32_t>(S_PASS, kMaxPasses, &E);
-    d.cell_start = h->get<int32_t>(S_CSTART, ntot + 1, &E);
-    d.cell_key = h->get<uint64_t>(S_CKEY, ntot, &E);
-    d.cell_xyz = h->get<uint64_t>(S_CXYZ, ntot, &E);
-    d.cell_z = h->get<int32_t>(S_CZ, ntot, &E);
-    d.cell_comp = h->get<int32_t>(S_CCOMP, ntot, &E);
+    RunMem& M = mem;
+    d.inst = M.get<Inst>(S_INST, ninst);
+    d.chunks = M.get<Chunk>(S_CHUNK, nchunks);
+    d.bbox = M.get<unsigned long long>(S_BBOX, (size_t)ninst * kBoxStripes * 8);
+    d.ctl = M.get<long long>(S_CTL, C_NCTL);
+    d.stat = M.get<long long>(S_STAT, (size_t)C_NCTL * kStripes);
+    d.key[0] = M.get<uint64_t>(S_KEY0, ntot);
+    d.key[1] = M.get<uint64_t>(S_KEY1, ntot);
+    d.val[0] = M.get<int32_t>(S_VAL0, ntot);
+    d.val[1] = M.get<int32_t>(S_VAL1, ntot);
+    d.hist = M.get<uint32_t>(S_HIST, sort_hist_entries(ntot));
+    d.offs = M.get<uint32_t>(S_OFFS, sort_hist_entries(ntot));
+    d.dtot = M.get<uint32_t>(S_DTOT, 256);
+    d.pass_scatter = M.get<int32_t>(S_PASS, kMaxPasses);
+    d.cell_start = M.get<int32_t>(S_CSTART, ntot + 1);
+    d.cell_key = M.get<uint64_t>(S_CKEY, ntot);
+    d.cell_xyz = M.get<uint64_t>(S_CXYZ, ntot);
+    d.cell_z = M.get<int32_t>(S_CZ, ntot);
+    d.cell_comp = M.get<int32_t>(S_CCOMP, ntot);
     // striped work lists: sub-list capacity = the most its workgroups append
     const int64_t cap_p = (point_blocks(ntot) + kStripes - 1) / kStripes * 256;
     const int64_t cap_c = (cell_comp_blocks(ntot) + kStripes - 1) / kStripes * 1024;
-    d.adj_pts = SList{h->get<int32_t>(S_ADJPTS, (size_t)cap_p * kStripes, &E),
+    d.adj_pts = SList{M.get<int32_t>(S_ADJPTS, (size_t)cap_p * kStripes),
                       (unsigned long long*)(d.stat + (int64_t)C_NADJPTS * kStripes), cap_p};
-    d.heavy = SList{h->get<int32_t>(S_HEAVY, (size_t)cap_p * kStripes, &E),
+    d.heavy = SList{M.get<int32_t>(S_HEAVY, (size_t)cap_p * kStripes),
                     (unsigned long long*)(d.stat + (int64_t)C_NHEAVY * kStripes), cap_p};
-    d.core_cells = SList{h->get<int32_t>(S_CORECELLS, (size_t)cap_c * kStripes, &E),
+    d.core_cells = SList{M.get<int32_t>(S_CORECELLS, (size_t)cap_c * kStripes),
                          (unsigned long long*)(d.stat + (int64_t)C_NCORECELLS * kStripes), cap_c};
-    d.cell_inst = h->get<int32_t>(S_CINST, ntot, &E);
-    d.cell_ncore = h->get<int32_t>(S_CM, ntot, &E);
-    d.rows = h->get<int32_t>(S_ROWS, (size_t)ntot * kMaxRows, &E);
-    d.cell_of = h->get<int32_t>(S_CELLOF, ntot, &E);
+    d.cell_inst = M.get<int32_t>(S_CINST, ntot);
+    d.cell_ncore = M.get<int32_t>(S_CM, ntot);
+    d.rows = M.get<int32_t>(S_ROWS, (size_t)ntot * kMaxRows);
+    d.cell_of = M.get<int32_t>(S_CELLOF, ntot);
     if (!f32) {  // the FP32 form reads FP64 coordinates from the caller's rows
-      d.X[0] = h->get<double>(S_X0, ntot, &E);
-      d.X[1] = h->get<double>(S_X1, ntot, &E);
-      d.X[2] = h->get<double>(S_X2, ntot, &E);
+      d.X[0] = M.get<double>(S_X0, ntot);
+      d.X[1] = M.get<double>(S_X1, ntot);
+      d.X[2] = M.get<double>(S_X2, ntot);
     } else {
-      d.xoff = h->get<int64_t>(S_XOFF, ntot, &E);
+      d.xoff = M.get<int64_t>(S_XOFF, ntot);
     }
-    d.G[0] = h->get<uint32_t>(S_G0, ntot, &E);
-    d.G[1] = h->get<uint32_t>(S_G1, ntot, &E);
-    d.G[2] = h->get<uint32_t>(S_G2, ntot, &E);
-    d.orig = h->get<int32_t>(S_ORIG, ntot, &E);
-    d.core = h->get<uint8_t>(S_CORE, ntot, &E);
-    d.cell_label = h->get<int32_t>(S_LABEL, ntot, &E);
-    d.amask = h->get<unsigned long long>(S_AMASK, (size_t)ntot, &E);
-    d.rec_n = h->get<uint8_t>(S_RECN, ntot, &E);
-    if (rec_stride > 0) d.rec = h->get<int32_t>(S_REC, (size_t)ntot * rec_stride, &E);
-    d.parent = h->get<int32_t>(S_PARENT, ntot, &E);
-    d.compmin = h->get<int32_t>(S_COMPMIN, ntot, &E);
-    d.pairs = h->get<int2>(S_PAIRS, shape.pair_cap, &E);
-    d.pair_cnt = h->get<int32_t>(S_PAIRCNT, shape.waves, &E);
-    d.seg_pos = h->get<int64_t>(S_SEGPOS, shape.waves, &E);
-    d.q[0] = h->get<int32_t>(S_Q0, ntot, &E);
-    d.q[1] = h->get<int32_t>(S_Q1, ntot, &E);
-    d.keyv = h->get<int32_t>(S_KEYV, ntot, &E);
-    d.flag = h->get<uint8_t>(S_FLAG, ntot, &E);
-    d.rank = h->get<int32_t>(S_RANK, ntot + 1, &E);
-    d.scan_tmp = h->get<int64_t>(S_SCAN, scan_tmp_entries(ntot), &E);
-    d.core_out = h->get<uint8_t>(S_COREOUT, ntot, &E);
-    if (herr != hipSuccess) return DCLUST_E_OOM;
+    d.G[0] = M.get<uint32_t>(S_G0, ntot);
+    d.G[1] = M.get<uint32_t>(S_G1, ntot);
+    d.G[2] = M.get<uint32_t>(S_G2, ntot);
+    d.orig = M.get<int32_t>(S_ORIG, ntot);
+    d.core = M.get<uint8_t>(S_CORE, ntot);
+    d.cell_label = M.get<int32_t>(S_LABEL, ntot);
+    d.amask = M.get<unsigned long long>(S_AMASK, (size_t)ntot);
+    d.rec_n = M.get<uint8_t>(S_RECN, ntot);
+    if (rec_stride > 0) d.rec = M.get<int32_t>(S_REC, (size_t)ntot * rec_stride);
+    d.parent = M.get<int32_t>(S_PARENT, ntot);
+    d.compmin = M.get<int32_t>(S_COMPMIN, ntot);
+    d.pairs = M.get<int2>(S_PAIRS, shape.pair_cap);
+    d.pair_cnt = M.get<int32_t>(S_PAIRCNT, shape.waves);
+    d.seg_pos = M.get<int64_t>(S_SEGPOS, shape.waves);
+    d.q[0] = M.get<int32_t>(S_Q0, ntot);
+    d.q[1] = M.get<int32_t>(S_Q1, ntot);
+    d.keyv = M.get<int32_t>(S_KEYV, ntot);
+    d.flag = M.get<uint8_t>(S_FLAG, ntot);
+    d.rank = M.get<int32_t>(S_RANK, ntot + 1);
+    d.scan_tmp = M.get<int64_t>(S_SCAN, scan_tmp_entries(ntot));
+    d.core_out = M.get<uint8_t>(S_COREOUT, ntot);
+    if (mem.over) return DCLUST_E_CAPACITY;  // the fixed region is too small
+    if (mem.err != hipSuccess) {
+      herr = mem.err;
+      return DCLUST_E_OOM;
+    }
   }
   d.labels_out = d_labels;
 
   // A call whose launch state equals the previous call's (sizes, parameters,
   // pointers) is captured once into a graph and replayed from then on.
   std::vector<uint8_t> sig;
-  const bool graph_on = !onestep && graph_enabled(h);
+  const bool graph_on = !onestep && !fixed_mem && graph_enabled(h);
   if (graph_on) sig = launch_signature(d, shape, f32, inst, chunks);
-  if (h->gexec && (!graph_on || sig != h->gsig)) drop_graph(h);
+  // (a run in a fixed region leaves the handle's graph alone: destroying it
+  // would free memory)
+  if (!fixed_mem && h->gexec && (!graph_on || sig != h->gsig)) drop_graph(h);
   if (graph_on && !h->gexec && !h->prev_sig.empty() && sig == h->prev_sig &&
       h->prev_npass < kGraphMaxPasses) {
     Dev dg = d;
@@ -600,7 +675,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
   {
     long long ctl_h[C_NCTL];
     std::vector<long long> stat_h((size_t)C_NCTL * kStripes);
-    if (h->gexec) {
+    if (h->gexec && !fixed_mem) {
       HIPCHK(hipMemcpyAsync(d.inst, inst.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, s));
       HIPCHK(hipMemcpyAsync((void*)d.chunks, chunks.data(), sizeof(Chunk) * nchunks,
                             hipMemcpyHostToDevice, s));
@@ -648,9 +723,15 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     // dense line table when it is small enough, else binary search
     d.line_cap = 0;
     if (nlines <= kMaxLineTable) {
-      d.lines = h->get<int2>(S_LINES, (size_t)std::max<long long>(nlines, 1), &herr);
-      if (!d.lines) return DCLUST_E_OOM;
-      d.line_cap = nlines;
+      d.lines = mem.get<int2>(S_LINES, (size_t)std::max<long long>(nlines, 1));
+      if (d.lines) {
+        d.line_cap = nlines;
+      } else if (!fixed_mem) {
+        herr = mem.err;
+        return DCLUST_E_OOM;
+      }
+      // (a fixed region without room for the table: the rows are found by
+      // binary search over the cell keys instead)
     }
     launch_index_tail(d, ntot, nchunks, grid_cells, passes, s);
     HIPCHK(mark(h, E_ROWS0, false));
@@ -913,6 +994,12 @@ int dclust_create(dclust_handle** out, int device) {
     return DCLUST_E_HIP;
   }
   h->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  // this device's scratch array for dclust_run_device_noalloc (the device is
+  // current); without it that entry point reports DCLUST_E_HIP
+  if (scratch_address(device, &h->scratch) != hipSuccess) {
+    (void)hipGetLastError();
+    h->scratch = nullptr;
+  }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return DCLUST_E_HIP;
@@ -944,6 +1031,16 @@ int dclust_run_device(dclust_handle* h, const double* d_xyz, int64_t n, const dc
   if (!h || !p || n < 0 || (n > 0 && (!d_xyz || !d_labels_out))) return DCLUST_E_PARAM;
   if (hipSetDevice(h->device) != hipSuccess) return DCLUST_E_HIP;
   return run_device(h, 1, d_xyz, &n, p, d_labels_out, nullptr, stats);
+}
+
+int dclust_run_device_noalloc(dclust_handle* h, const double* d_xyz, int64_t n,
+                              const dclust_params* p, int32_t* d_labels_out,
+                              dclust_stats* stats) {
+  if (!h || !p || n < 0 || (n > 0 && (!d_xyz || !d_labels_out))) return DCLUST_E_PARAM;
+  if (!h->scratch) return DCLUST_E_HIP;
+  if (hipSetDevice(h->device) != hipSuccess) return DCLUST_E_HIP;
+  return run_device(h, 1, d_xyz, &n, p, d_labels_out, nullptr, stats, 0, h->scratch,
+                    kScratchBytes);
 }
 
 int dclust_run_onestep_device(dclust_handle* h, const double* d_xyz, int64_t n,

@@ -108,6 +108,8 @@ def _lib():
                                                 C.POINTER(Stats)]
         L.dclust_run_onestep_device.argtypes = [_P, _P, C.c_int64, C.POINTER(Params), _P,
                                                 C.POINTER(Stats)]
+        L.dclust_run_device_noalloc.argtypes = [_P, _P, C.c_int64, C.POINTER(Params), _P,
+                                                C.POINTER(Stats)]
         L.dclust_last_core_flags.argtypes = [_P, _P, C.c_int64]
         L.dclust_autotune.argtypes = [_P, C.c_int64, C.POINTER(Params)]
         L.dclust_autotune_device.argtypes = [_P, C.c_int32, _P, _P, _P]
@@ -220,6 +222,17 @@ class Handle:
         _check(_lib().dclust_run_device(self._h, C.c_void_p(d_xyz), int(n), C.byref(params),
                                         C.c_void_p(d_labels), C.byref(st)),
                "dclust_run_device")
+        self.last_stats = st.as_dict()
+        return self.last_stats
+
+    def run_device_noalloc(self, d_xyz: int, n: int, params: Params, d_labels: int):
+        """dclust_run_device_noalloc: the same run in the library's fixed
+        per-GPU scratch array (no allocation)."""
+        st = Stats()
+        _check(_lib().dclust_run_device_noalloc(self._h, C.c_void_p(d_xyz), int(n),
+                                                C.byref(params), C.c_void_p(d_labels),
+                                                C.byref(st)),
+               "dclust_run_device_noalloc")
         self.last_stats = st.as_dict()
         return self.last_stats
 

@@ -180,6 +180,22 @@ int dclust_run_device(dclust_handle* h, const double* d_xyz, int64_t n,
                       const dclust_params* p, int32_t* d_labels_out,
                       dclust_stats* stats);
 
+/* dclust_run_device without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, labels and statistics as dclust_run_device; every buffer
+ * of the run comes from one 64 MiB array per GPU that the library's code
+ * object carries, so the call makes no hipMalloc / hipFree.  Limit: the run
+ * must fit that array -- about 230,000 3-D points in the FP32 form with
+ * minpts <= 10 (about 210,000 in the FP64 form or with minpts >= 16, more
+ * in 2-D); larger inputs return DCLUST_E_CAPACITY before any work.  The run is eager (no graph
+ * replay).  All calls on one GPU share its array and each leaves its
+ * contents there (dclust_last_core_flags reads them): calls that overlap on
+ * one GPU, from any handles or threads, race. */
+int dclust_run_device_noalloc(dclust_handle* h, const double* d_xyz, int64_t n,
+                              const dclust_params* p, int32_t* d_labels_out,
+                              dclust_stats* stats);
+
 /* One-step expansion baseline: CUDA-DClust as the reference ships it
  * (cuda-dclust/dbscan_indexing.cu: kernel DBSCAN :529-637, host loop
  * :442-472, MonitorSeedPoints :639-781), on this library's grid index.
