@@ -221,9 +221,17 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
   int nrec = 0;
   int ev = 0;
   const RowsOf rows = rows_of(d, c);
+  // the next group's packed rows are loaded while this group streams
+  int32_t pkn[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) pkn[j] = rows[j];
   for (int g = 0; g < NR && cnt < P.minpts; g += 5) {
     int32_t aa[5], ll[5];
-    line_group(d, rows, g, aa, ll);
+    line_ranges(d, pkn, aa, ll);
+    if (g + 5 < NR) {
+#pragma unroll
+      for (int j = 0; j < 5; ++j) pkn[j] = rows[g + 5 + j];
+    }
     ++t.groups;
 #pragma unroll
     for (int j = 0; j < 5; ++j) t.lines += ll[j] > 0;
@@ -291,7 +299,7 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
 }
 
 template <bool F32>
-__global__ __launch_bounds__(256) void k_core(Dev d) {
+__global__ __launch_bounds__(256, 5) void k_core(Dev d) {
   if (abandoned(d)) return;
   const int64_t pl = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const bool act = pl < d.n;
@@ -476,11 +484,14 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
   for (int k = 0; k < D; ++k) fr[k] = (int32_t)(pg[k] & kFrac);
   const int32_t zA = act ? d.cell_z[c] : 0;
   const RowsOf rows = rows_of(d, c);
+  // the next line's packed row is loaded while this line is examined
+  int32_t pk_next = act ? rows[rank % NN] : 0;
   for (int step = 0; step < NN; ++step) {
     const int r = (step + rank) % NN;
     const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
     const int o1 = D == 3 ? kRowOff3[r][1] : 0;
-    const int32_t pk = act ? rows[r] : 0;
+    const int32_t pk = pk_next;
+    if (step + 1 < NN) pk_next = act ? rows[(step + 1 + rank) % NN] : 0;
     const int32_t len = pk >> kRowCntShift;
     if (len > 0) {
       ++t.lines;
@@ -809,41 +820,43 @@ __device__ long long far_cell(const Dev& d, const Inst& I, const Par& P, int32_t
   const int32_t qs = d.cell_start[A], qe = d.cell_start[A + 1];
   const bool allc = d.cell_ncore[A] == qe - qs;
   const int32_t compA = d.cell_comp[A];
+  const int32_t zA = d.cell_z[A];
   const RowsOf rows = rows_of(d, A);
   long long joins = 0;
-  for (int r = 0; r < NR; ++r) {
-    const int32_t pk = rows[r];
-    const int32_t len = pk >> kRowCntShift;
-    if (len == 0) continue;
-    const int32_t st = pk & kRowCellMask;
-    if (st + len - 1 <= A) continue;  // no cell above A in this line
-    const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
-    const int o1 = D == 3 ? kRowOff3[r][1] : 0;
-    const bool far_l = o0 == 2 || o0 == -2 || o1 == 2 || o1 == -2;
-    // cells above A in another component (cell_comp is -1 for a cell
-    // without core points); in a near line only the cells two steps away
-    // along the line are far
+  // five lines at a time: their packed rows in one round of loads, then the
+  // components of all their cells above A in the next; bit 5j+k of `todo`
+  // marks cell k of line j as above A in another component (cell_comp is
+  // -1 for a cell without core points)
+  for (int g = 0; g < NR; g += 5) {
+    int32_t pk[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) pk[j] = rows[g + j];
     unsigned todo = 0u;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      const int32_t B = st + k;
-      const bool up = k < len && B > A;
-      const int32_t comp = up ? d.cell_comp[B] : compA;
-      if (comp >= 0 && comp != compA) todo |= 1u << k;
-    }
-    if (todo && !far_l) {
+    for (int j = 0; j < 5; ++j) {
+      const int32_t len = pk[j] >> kRowCntShift, st = pk[j] & kRowCellMask;
 #pragma unroll
-      for (int k = 0; k < 5; ++k)
-        if (todo & (1u << k)) {
-          const int ol = cell_offset_last<D>(d, I, A, st + k, r);
-          if (ol != 2 && ol != -2) todo &= ~(1u << k);
-        }
+      for (int k = 0; k < 5; ++k) {
+        const int32_t comp = (k < len && st + k > A) ? d.cell_comp[st + k] : compA;
+        if (comp >= 0 && comp != compA) todo |= 1u << (5 * j + k);
+      }
     }
     while (todo) {
-      const int k = __ffs(todo) - 1;
+      const int b = __ffs(todo) - 1;
       todo &= todo - 1;
-      const int32_t B = st + k;
-      const int ol = cell_offset_last<D>(d, I, A, B, r);
+      const int j = (b * 205) >> 10;  // b / 5 for b < 25
+      const int k = b - 5 * j;
+      const int r = g + j;
+      int32_t pkj = pk[0];
+#pragma unroll
+      for (int q = 1; q < 5; ++q) pkj = j == q ? pk[q] : pkj;
+      const int32_t B = (pkj & kRowCellMask) + k;
+      const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
+      const int o1 = D == 3 ? kRowOff3[r][1] : 0;
+      const bool far_l = o0 == 2 || o0 == -2 || o1 == 2 || o1 == -2;
+      const int ol = d.cell_z[B] - zA;  // B lies in a neighbour line of A
+      // in a near line only the cells two steps away along the line are far
+      if (!far_l && ol != 2 && ol != -2) continue;
       if (cells_joined<D, F32>(d, P, qs, qe, allc, d.cell_start[B], d.cell_start[B + 1], o0, o1,
                                ol, t, freeb)) {
         uf_union(d.parent, compA, d.cell_comp[B]);
@@ -851,11 +864,12 @@ __device__ long long far_cell(const Dev& d, const Inst& I, const Par& P, int32_t
       }
     }
   }
+  (void)I;
   return joins;
 }
 
 template <bool F32>
-__global__ __launch_bounds__(256) void k_far(Dev d) {
+__global__ __launch_bounds__(256, 5) void k_far(Dev d) {
   if (abandoned(d)) return;
   __shared__ long long s_pre[kStripes + 1];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

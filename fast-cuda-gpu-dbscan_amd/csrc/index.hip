@@ -361,8 +361,8 @@ __global__ __launch_bounds__(RS_T) void k_rs_hist(Dev d, int pass, int nblk) {
   d.hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// Digit totals (one block per digit), then digit bases + pass decision,
-// then per-digit row scans.  offs[d][b] = global start of digit d in block b.
+// Digit totals, one block per digit (a sum over the digit's row of block
+// histograms; atomics from every block into 256 totals were slower).
 __global__ __launch_bounds__(256) void k_rs_total(Dev d, int pass, int nblk) {
   if (!pass_active(d, pass)) return;
   const uint32_t* row = d.hist + (int64_t)blockIdx.x * nblk;
@@ -375,46 +375,41 @@ __global__ __launch_bounds__(256) void k_rs_total(Dev d, int pass, int nblk) {
   if (threadIdx.x == 0) d.dtot[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-__global__ __launch_bounds__(256) void k_rs_base(Dev d, int pass) {
-  __shared__ uint32_t s[256];
-  __shared__ int s_skip;
-  if (!pass_active(d, pass)) {
-    if (threadIdx.x == 0) d.pass_scatter[pass] = 0;
-    return;
-  }
-  uint32_t v = d.dtot[threadIdx.x];
-  if (threadIdx.x == 0) s_skip = 0;
-  __syncthreads();
-  if ((int64_t)v == d.n) s_skip = 1;  // every key has this digit
-  s[threadIdx.x] = v;
-  __syncthreads();
-  for (int o = 1; o < 256; o <<= 1) {
-    uint32_t u = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
-    __syncthreads();
-    s[threadIdx.x] += u;
-    __syncthreads();
-  }
-  d.dtot[threadIdx.x] = s[threadIdx.x] - v;  // exclusive digit base
-  if (threadIdx.x == 0) d.pass_scatter[pass] = s_skip ? 0 : 1;
-}
-
+// Per-digit row scans, one block per digit: offs[d][b] = global start of
+// digit d in block b.  Every block also derives its digit's base (the keys
+// of all smaller digits) and the pass decision from the digit totals: a
+// pass in which every key has the same digit
+// leaves the data in order, so its scatter is skipped and the buffers are
+// not swapped (block 0 records the decision for the scatter and for the
+// buffer parity of later passes).
 __global__ __launch_bounds__(256) void k_rs_rowscan(Dev d, int pass, int nblk) {
-  if (!pass_active(d, pass) || !d.pass_scatter[pass]) return;
-  const uint32_t* row = d.hist + (int64_t)blockIdx.x * nblk;
-  uint32_t* out = d.offs + (int64_t)blockIdx.x * nblk;
+  const uint32_t* tot = d.dtot;
   __shared__ uint32_t s_w[4];
   __shared__ uint32_t s_carry;
-  if (threadIdx.x == 0) s_carry = d.dtot[blockIdx.x];
+  __shared__ int s_skip;
+  const bool active = pass_active(d, pass);
+  const uint32_t v = tot[threadIdx.x];
+  if (threadIdx.x == 0) s_skip = 0;
   __syncthreads();
+  if (active && (int64_t)v == d.n) s_skip = 1;  // every key has this digit
+  const uint32_t below = wave_sum(threadIdx.x < blockIdx.x ? v : 0u);
+  if (lane_id() == 0) s_w[threadIdx.x >> 6] = below;
+  __syncthreads();
+  if (threadIdx.x == 0) s_carry = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.pass_scatter[pass] = (active && !s_skip) ? 1 : 0;
+  __syncthreads();
+  if (!active || s_skip) return;
+  const uint32_t* row = d.hist + (int64_t)blockIdx.x * nblk;
+  uint32_t* out = d.offs + (int64_t)blockIdx.x * nblk;
   for (int b0 = 0; b0 < nblk; b0 += blockDim.x) {
     int b = b0 + threadIdx.x;
-    uint32_t v = b < nblk ? row[b] : 0;
-    uint32_t inc = wave_incl_scan(v);
+    uint32_t x = b < nblk ? row[b] : 0;
+    uint32_t inc = wave_incl_scan(x);
     if (lane_id() == 63) s_w[threadIdx.x >> 6] = inc;
     __syncthreads();
     uint32_t pre = s_carry;
     for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) pre += s_w[w];
-    if (b < nblk) out[b] = pre + inc - v;
+    if (b < nblk) out[b] = pre + inc - x;
     __syncthreads();
     if (threadIdx.x == 0) s_carry += s_w[0] + s_w[1] + s_w[2] + s_w[3];
     __syncthreads();
@@ -711,7 +706,9 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
   if (abandoned(d)) return;
   const int32_t C = (int32_t)d.ctl[C_NCELL];
   const bool table = d.ctl[C_NLINES] <= d.line_cap;
-  const int g = blockIdx.y;  // lines 5g .. 5g+4 of the stencil
+  // 3-D: the five lines with offset g - 2 along axis 0 (consecutive entries
+  // of the line table); 2-D: the five lines (g = 0)
+  const int g = blockIdx.y;
   long long cand = 0, nbrc = 0;
   // one lane per cell, consecutive cells in consecutive lanes: they mostly
   // share their neighbour lines (broadcast loads); each lane has five
@@ -729,13 +726,12 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
     bool need = false;
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
-      const int r = 5 * g + j;
       int o0 = 0, o1 = 0;
       if (dim == 3) {
-        o0 = kRowOff3[r][0];
-        o1 = kRowOff3[r][1];
+        o0 = g - 2;
+        o1 = j - 2;
       } else {
-        o0 = r < 5 ? kRowOff2[r] : 0;
+        o0 = kRowOff2[j];
       }
       const int32_t r0 = kc[0] + o0, r1 = kc[1] + o1;
       bool ok = (dim == 3 || g == 0) && r0 >= 0 && r0 < I.nk[0];
@@ -831,7 +827,11 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
       }
     }
 #pragma unroll
-    for (int j = 0; j < 5; ++j) d.rows[(int64_t)(5 * g + j) * d.rstride + c] = packed[j];
+    for (int j = 0; j < 5; ++j) {
+      // (2-D: groups g > 0 write the unused rows 5g .. 5g+4, empty)
+      const int r = dim == 3 ? kRowOfOff3[g][j] : 5 * g + j;
+      d.rows[(int64_t)r * d.rstride + c] = packed[j];
+    }
   }
   stat_add(d, C_CAND, cand);
   stat_add(d, C_NBRC, nbrc);
@@ -857,7 +857,6 @@ void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
   for (int p = 0; p < passes; ++p) {
     hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
     hipLaunchKernelGGL(k_rs_total, dim3(256), dim3(256), 0, s, d, p, nblk);
-    hipLaunchKernelGGL(k_rs_base, dim3(1), dim3(256), 0, s, d, p);
     hipLaunchKernelGGL(k_rs_rowscan, dim3(256), dim3(256), 0, s, d, p, nblk);
     hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
   }

@@ -41,21 +41,30 @@ constexpr int kRecSlots = 16;  // neighbour records kept per non-core point (at 
    {1, -2},  {-1, 2}, {1, 2},   {-2, -2}, {-2, 2}, {2, -2}, {2, 2}}
 #define DCLUST_ROW_MIRROR3 \
   {0, 2, 1, 4, 3, 8, 7, 6, 5, 10, 9, 12, 11, 16, 15, 14, 13, 20, 19, 18, 17, 24, 23, 22, 21}
+// row of the line with offsets (a - 2, b - 2) along axes 0 and 1 (3-D)
+#define DCLUST_ROW_OF_OFF3                                                     \
+  {{21, 13, 9, 14, 22}, {17, 5, 1, 6, 19}, {11, 3, 0, 4, 12}, {18, 7, 2, 8, 20}, \
+   {23, 15, 10, 16, 24}}
 #define DCLUST_ROW_OFF2 {0, -1, 1, -2, 2}
 #define DCLUST_ROW_MIRROR2 {0, 2, 1, 4, 3}
 static __constant__ int8_t kRowOff3[25][2] = DCLUST_ROW_OFF3;
 static __constant__ int8_t kRowOff2[5] = DCLUST_ROW_OFF2;
 static __constant__ int8_t kRowMirror3[25] = DCLUST_ROW_MIRROR3;
 static __constant__ int8_t kRowMirror2[5] = DCLUST_ROW_MIRROR2;
+static __constant__ int8_t kRowOfOff3[5][5] = DCLUST_ROW_OF_OFF3;
 
 namespace tables {
 constexpr int8_t off3[25][2] = DCLUST_ROW_OFF3;
 constexpr int8_t mir3[25] = DCLUST_ROW_MIRROR3;
 constexpr int8_t off2[5] = DCLUST_ROW_OFF2;
 constexpr int8_t mir2[5] = DCLUST_ROW_MIRROR2;
+constexpr int8_t rof3[5][5] = DCLUST_ROW_OF_OFF3;
 constexpr bool mirrors_ok() {
   for (int r = 0; r < 25; ++r)
     if (off3[mir3[r]][0] != -off3[r][0] || off3[mir3[r]][1] != -off3[r][1]) return false;
+  for (int a = 0; a < 5; ++a)
+    for (int b = 0; b < 5; ++b)
+      if (off3[rof3[a][b]][0] != a - 2 || off3[rof3[a][b]][1] != b - 2) return false;
   for (int r = 0; r < 5; ++r)
     if (off2[mir2[r]] != -off2[r]) return false;
   return true;
@@ -208,7 +217,7 @@ struct Dev {
   int32_t* val[2];
   uint32_t* hist;   // [256][nblk] digit-major block histograms
   uint32_t* offs;   // [256][nblk] scanned offsets
-  uint32_t* dtot;   // [256]
+  uint32_t* dtot;   // [256] digit totals of the current pass
   int32_t* pass_scatter;  // [kMaxPasses] 1 if the scatter of pass p ran
 
   // CSR

@@ -83,12 +83,9 @@ struct Stream5 {
   }
 };
 
-// point ranges of five stencil lines
-__device__ __forceinline__ void line_group(const Dev& d, const RowsOf& rows, int g, int32_t* a,
-                                           int32_t* l) {
-  int32_t pk[5];
-#pragma unroll
-  for (int j = 0; j < 5; ++j) pk[j] = rows[g + j];
+// point ranges of five stencil lines given their packed rows
+__device__ __forceinline__ void line_ranges(const Dev& d, const int32_t* pk, int32_t* a,
+                                            int32_t* l) {
 #pragma unroll
   for (int j = 0; j < 5; ++j) {
     const int32_t len = pk[j] >> kRowCntShift, st = pk[j] & kRowCellMask;
@@ -100,6 +97,15 @@ __device__ __forceinline__ void line_group(const Dev& d, const RowsOf& rows, int
     a[j] = x;
     l[j] = y - x;
   }
+}
+
+// point ranges of five stencil lines
+__device__ __forceinline__ void line_group(const Dev& d, const RowsOf& rows, int g, int32_t* a,
+                                           int32_t* l) {
+  int32_t pk[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) pk[j] = rows[g + j];
+  line_ranges(d, pk, a, l);
 }
 
 }  // namespace dclust
