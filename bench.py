@@ -290,11 +290,12 @@ BATCH_DESC = ("cfg4: 4,096 independent 2-D/3-D instances, sizes log-uniform 200-
               "minpts in {4,8,16}, FP32 + guard band")
 
 
-def batched_leg(hip, h, steps, warmup, check_oracle=True, count=4096):
-    """One batched clustering of every instance (autotuned launch shape)
-    against one plain single-instance call per instance (library default
-    shape), on the same device-resident inputs; labels compared instance by
-    instance, and a sample checked against the oracle."""
+def batched_leg(hip, h, steps, warmup, check_oracle=True, count=4096, online_tune=False):
+    """One batched clustering of every instance against one plain
+    single-instance call per instance, on the same device-resident inputs;
+    labels compared instance by instance, and a sample checked against the
+    oracle.  Both use the library's built-in shape table; with online_tune
+    the batch runs with the shape dclust_autotune_device measures instead."""
     from concurrent.futures import ThreadPoolExecutor
 
     with ThreadPoolExecutor(max_workers=16) as ex:
@@ -311,9 +312,12 @@ def batched_leg(hip, h, steps, warmup, check_oracle=True, count=4096):
     hip.h2d(d_x, flat)
     d_b = hip.malloc(tot * 4)
     d_1 = hip.malloc(tot * 4)
-    t0 = time.perf_counter()
-    tuned = h.autotune_device(d_x, ns, plain)
-    t_tune = time.perf_counter() - t0
+    t_tune = None
+    tuned = plain
+    if online_tune:
+        t0 = time.perf_counter()
+        tuned = h.autotune_device(d_x, ns, plain)
+        t_tune = time.perf_counter() - t0
     for _ in range(warmup):
         h.run_batched_device(d_x, ns, tuned, d_b)
     hip.sync()
@@ -345,9 +349,10 @@ def batched_leg(hip, h, steps, warmup, check_oracle=True, count=4096):
            "per_instance_mpoints_s": round(tot / t_pi / 1e6, 2),
            "instance_errors": int(np.count_nonzero(errs)),
            "instances_differing_batched_vs_per_instance": len(bad),
-           "tuned_shape": {"seed_list_size": tuned[0].seed_list_size,
-                           "blocks": tuned[0].blocks, "block_threads": tuned[0].block_threads},
-           "autotune_s": round(t_tune, 2),
+           "shape": {"seed_list_size": st_b["eff_seed_list_size"], "blocks": st_b["eff_blocks"],
+                     "block_threads": st_b["eff_block_threads"],
+                     "source": "online sweep" if online_tune else "built-in table"},
+           "autotune_s": round(t_tune, 2) if t_tune is not None else None,
            "phases_ms": {k: round(st_b[k], 3) for k in st_b if k.startswith("t_")}}
     if check_oracle:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -487,6 +492,76 @@ def summarize_profile(path, skip_runs, traffic=None):
     return "\n".join(out)
 
 
+def tune_table(hip, h, reps=10):
+    """Measures the launch shapes behind the library's built-in shape table:
+    dclust_autotune_device's sweep (seed-list length x workgroup size x
+    workgroups per CU) on representative problems of each dimension mix and
+    size class, each best shape timed against the library's default shape;
+    then clusterings of the same points at several PARTITION_SIZE values
+    (r only validates the bin width: the grid is fixed by eps)."""
+    rows = []
+
+    def timed(d_x, ns, params, d_l):
+        for _ in range(2):
+            h.run_batched_device(d_x, ns, params, d_l)
+        hip.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            h.run_batched_device(d_x, ns, params, d_l)
+        hip.sync()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    cases = [("cfg1", 1, 0), ("cfg2 100k", 2, 100_000), ("cfg2", 2, 0), ("cfg3 100k", 3, 100_000),
+             ("cfg3 400k", 3, 400_000), ("cfg3", 3, 0), ("cfg5 4M", 5, 4_000_000), ("cfg5", 5, 0),
+             ("cfg4 first 256", 4, 256), ("cfg4", 4, 4096)]
+    for name, cfg, n in cases:
+        if cfg == 4:
+            xs, infos = zip(*[dclust.synth_batch_instance(i) for i in range(n)])
+        else:
+            x, info = dclust.synth_config(cfg, n=n)
+            xs, infos = [x], [info]
+        ns = np.array([x.shape[0] for x in xs], dtype=np.int64)
+        flat = np.concatenate([x.ravel() for x in xs])
+        d_x, d_l = hip.malloc(flat.nbytes), hip.malloc(int(ns.sum()) * 4)
+        hip.h2d(d_x, flat)
+        base = [dclust.make_params(i["dim"], i["eps"], i["minpts"], precision=dclust.FP32_GUARD)
+                for i in infos]
+        t_def = timed(d_x, ns, base, d_l)
+        tuned = h.autotune_device(d_x, ns, base)
+        t_best = timed(d_x, ns, tuned, d_l)
+        dims = sorted({int(i["dim"]) for i in infos})
+        row = {"case": name, "dims": dims, "points": int(ns.sum()),
+               "log2_points": int(np.floor(np.log2(max(1, ns.sum())))),
+               "best": {"seed_list_size": tuned[0].seed_list_size,
+                        "block_threads": tuned[0].block_threads, "blocks": tuned[0].blocks},
+               "ms_default_shape": round(t_def, 4), "ms_best_shape": round(t_best, 4)}
+        log(f"tune {row}")
+        rows.append(row)
+        hip.free(d_x)
+        hip.free(d_l)
+    # PARTITION_SIZE: the same clustering at several r (it validates the bin
+    # width only; the grid is fixed by eps)
+    for cfg, rs in ((1, (0, 1, 2, 5)), (3, (0, 1, 10, 100, 500))):
+        x, info = dclust.synth_config(cfg)
+        d_x, d_l = hip.malloc(x.nbytes), hip.malloc(x.shape[0] * 4)
+        hip.h2d(d_x, x)
+        ref = None
+        for r in rs:
+            p = dclust.make_params(x.shape[1], info["eps"], info["minpts"], r=r,
+                                   precision=dclust.FP32_GUARD)
+            t = timed(d_x, np.array([x.shape[0]], dtype=np.int64), [p], d_l)
+            lab = np.empty(x.shape[0], dtype=np.int32)
+            hip.d2h(lab, d_l)
+            ref = lab if ref is None else ref
+            row = {"case": f"cfg{cfg} r={r}", "ms": round(t, 4),
+                   "labels_equal_r0": bool(np.array_equal(lab, ref))}
+            log(f"r {row}")
+            rows.append(row)
+        hip.free(d_x)
+        hip.free(d_l)
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -513,6 +588,12 @@ def main():
                          "(FETCH_SIZE + WRITE_SIZE, calibrated) as JSON")
     ap.add_argument("--skip-runs", type=int, default=1,
                     help="runs left out of --summarize-profile (the eager warm-up)")
+    ap.add_argument("--online-tune", action="store_true",
+                    help="batched leg: run dclust_autotune_device's sweep instead of the "
+                         "built-in shape table")
+    ap.add_argument("--tune-table", action="store_true",
+                    help="measure the launch shapes of the built-in shape table and the "
+                         "effect of PARTITION_SIZE, print them as JSON and exit")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (a plain copy of known size, "
                          "used to calibrate the HBM byte counters of a profiler run)")
@@ -539,6 +620,12 @@ def main():
               "(how the path would shard is written up with the design notes).", file=sys.stderr)
         sys.exit(2)
 
+    if args.tune_table:
+        hip = Hip()
+        hip.check(hip.L.hipSetDevice(0), "hipSetDevice")
+        with dclust.Handle(0) as h:
+            print(json.dumps(tune_table(hip, h)), flush=True)
+        return
     cfg, desc = WORKLOADS[args.workload]
     x, info = dclust.synth_config(cfg)
     n, dim = x.shape
@@ -674,7 +761,8 @@ def main():
     if not args.no_batched:
         log("batched leg")
         out["batched"] = batched_leg(hip, h, max(2, args.steps // 2), 1,
-                                     check_oracle=not args.no_parity)
+                                     check_oracle=not args.no_parity,
+                                     online_tune=args.online_tune)
     if not args.no_onestep:
         log("one-step (CUDA-DClust) baseline")
         out["onestep_baseline"] = onestep_leg(hip, h, x, info, prec)

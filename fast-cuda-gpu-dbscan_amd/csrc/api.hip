@@ -216,14 +216,47 @@ int effective_threads(int t) {
   return std::min(256, (t + 63) / 64 * 64);
 }
 
-// Launch shape: explicit parameters win; zeros come from the tuning rule
-// (seed list 1024 entries, 256 threads, blocks at resident occupancy).
-RunShape pick_shape(dclust_handle* h, const dclust_params& p, int64_t n, bool f32) {
+// Built-in launch shapes of the expansion kernel, by the dimension mix of
+// the call and its total point count: the fastest of dclust_autotune_device's
+// 24 candidate shapes on one MI355X (256 CUs) for representative problems of
+// each class (`bench.py --tune-table`; the measurements are recorded with
+// the design notes).  One expansion launch serves every instance of a batch,
+// so a batch is one problem of its total size.
+struct ShapeRow {
+  int dims;       // 1: 2-D only, 2: 3-D only, 3: both
+  int log2_max;   // rows apply up to this floor(log2(points))
+  int seeds, threads, per_cu;
+};
+constexpr ShapeRow kShapeTable[] = {
+    {1, 16, 256, 128, 4},  {1, 19, 2048, 128, 4}, {1, 62, 1024, 256, 4},
+    {2, 16, 256, 128, 4},  {2, 19, 2048, 128, 4}, {2, 22, 1024, 256, 4}, {2, 62, 1024, 128, 4},
+    {3, 22, 1024, 256, 8}, {3, 62, 512, 256, 8},
+};
+
+const ShapeRow* shape_row(int dims, int64_t n) {
+  int lg = 0;
+  while (lg < 62 && (1LL << (lg + 1)) <= n) ++lg;
+  for (const ShapeRow& r : kShapeTable)
+    if (r.dims == dims && lg <= r.log2_max) return &r;
+  return nullptr;
+}
+
+// Launch shape: explicit parameters win; a call that leaves seed list,
+// workgroup size and workgroup count all at 0 takes the built-in table's
+// shape for its dimension mix and size, other zeros the rule (seed list 1024
+// entries, 256 threads, blocks at resident occupancy, at most 8 per CU).
+RunShape pick_shape(dclust_handle* h, const dclust_params& p, int64_t n, bool f32, int dims) {
   RunShape r;
-  r.seeds = p.seed_list_size > 0 ? p.seed_list_size : 1024;
-  r.threads = effective_threads(p.block_threads);
+  const ShapeRow* row = (p.seed_list_size == 0 && p.block_threads == 0 && p.blocks == 0)
+                            ? shape_row(dims, n)
+                            : nullptr;
+  r.seeds = p.seed_list_size > 0 ? p.seed_list_size : (row ? row->seeds : 1024);
+  r.threads = row ? row->threads : effective_threads(p.block_threads);
   if (p.blocks > 0) {
     r.blocks = p.blocks;
+  } else if (row) {
+    // as measured: workgroups beyond those resident start as others finish
+    r.blocks = h->num_cu * row->per_cu;
   } else {
     int per_cu = expand_blocks_per_cu(r.seeds, r.threads);
     r.blocks = h->num_cu * std::min(per_cu, 8);
@@ -531,7 +564,9 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
   if (stats) stats->n_points = ntot;
   if (ntot == 0) return DCLUST_OK;
 
-  RunShape shape = pick_shape(h, params[0], ntot, f32);
+  int dims = 0;
+  for (int32_t i = 0; i < ninst; ++i) dims |= params[i].dim == 3 ? 2 : 1;
+  RunShape shape = pick_shape(h, params[0], ntot, f32, dims);
   if (fixed_mem && params[0].extra_collision == 0) {
     // collision pairs beyond the per-wave buffer are merged inline: a
     // smaller buffer leaves more of the fixed region to the points
@@ -1238,7 +1273,7 @@ int dclust_autotune(dclust_handle* h, int64_t n, dclust_params* p) {
     if (p->blocks == 0) p->blocks = it->second.blocks;
     return DCLUST_OK;
   }
-  RunShape r = pick_shape(h, *p, n, f32);
+  RunShape r = pick_shape(h, *p, n, f32, p->dim == 3 ? 2 : 1);
   if (p->seed_list_size == 0) p->seed_list_size = r.seeds;
   if (p->block_threads == 0) p->block_threads = r.threads;
   if (p->blocks == 0) p->blocks = r.blocks;

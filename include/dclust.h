@@ -242,7 +242,9 @@ int dclust_last_core_flags(dclust_handle* h, uint8_t* core_out, int64_t n);
 /* Launch shape for a problem of n points (seed list, blocks, threads,
  * collision buffer): fills the zero fields of *p from the shapes that
  * dclust_autotune_device has measured on this handle for the same dimension,
- * size class (log2 n) and precision, else from the library's rule. */
+ * size class (log2 n) and precision, else from the library's built-in table
+ * (shapes measured on an MI355X per dimension mix and size class; a run
+ * whose seed list, blocks and threads are all 0 uses it too). */
 int dclust_autotune(dclust_handle* h, int64_t n, dclust_params* p);
 
 /* Empirical autotuner (no counterpart in the reference, whose launch shape is
@@ -251,8 +253,11 @@ int dclust_autotune(dclust_handle* h, int64_t n, dclust_params* p);
  * with every candidate shape -- seed-list length {256, 512, 1024, 2048} x
  * workgroup size {128, 256} x workgroups per CU {4, 8, 16} -- keeps the
  * fastest, writes it into params[0..count) and remembers it for
- * dclust_autotune.  The grid does not depend on PARTITION_SIZE (the cell
- * width is fixed by eps), so r is not a tuning dimension here. */
+ * dclust_autotune.  Optional: the built-in table holds the shapes this sweep
+ * found for representative problems.  The grid does not depend on
+ * PARTITION_SIZE (the cell width is fixed by eps; r only validates the bin
+ * width), so r is not a tuning dimension: labels and times are the same for
+ * every valid r. */
 int dclust_autotune_device(dclust_handle* h, int32_t count, const double* d_xyz,
                            const int64_t* n, dclust_params* params);
 

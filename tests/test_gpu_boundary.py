@@ -251,3 +251,44 @@ def test_fma_sensitive_pairs_follow_the_unfused_predicate(handle, dim):
         lab = handle.run(x, eps, 2, precision=prec)
         assert np.array_equal(lab, ref), f"precision {prec}"
         assert np.array_equal(handle.core_flags(2 * n), core)
+
+
+# ---- launch shape: built-in table, and PARTITION_SIZE ----
+@pytest.mark.parametrize("case", ["2d_small", "3d_mid", "mixed_batch"])
+def test_auto_launch_shape_comes_from_the_table(handle, case):
+    # the built-in table (shapes measured per dimension mix and size class):
+    # <= 2^16 points of one dimension -> 256 seeds x 128 threads; 2^18-2^19
+    # of one dimension -> 2048 x 128; a 2-D/3-D batch below 2^23 -> 1024 x 256
+    rng = np.random.default_rng(51)
+    if case == "2d_small":
+        x = rng.normal(0, 1, (20000, 2))
+        lab = handle.run(x, 0.05, 4)
+        want = (256, 128)
+        assert np.array_equal(lab, oracle.fast_dbscan(x, 0.05, 4)[0])
+    elif case == "3d_mid":
+        x, info = dclust.synth_config(3, n=300000)
+        lab = handle.run(x, info["eps"], info["minpts"])
+        want = (2048, 128)
+        assert np.array_equal(lab, oracle.fast_dbscan(x, info["eps"], info["minpts"])[0])
+    else:
+        xs = [rng.normal(0, 1, (5000, 2)), rng.normal(0, 1, (5000, 3))]
+        ps = [dclust.make_params(2, 0.05, 4), dclust.make_params(3, 0.15, 4)]
+        labs, errs = handle.run_batched(xs, ps)
+        assert not np.any(errs)
+        want = (1024, 256)
+        for x, p, lab in zip(xs, ps, labs):
+            assert np.array_equal(lab, oracle.fast_dbscan(x, p.eps, p.minpts)[0])
+    st = handle.last_stats
+    assert (st["eff_seed_list_size"], st["eff_block_threads"]) == want
+    assert st["eff_blocks"] > 0
+
+
+def test_partition_size_changes_nothing(handle, golden):
+    # r only validates the bin width (main.cu:204); the grid is fixed by eps
+    x, info = dclust.synth_config(1)
+    ref = golden["cfg1_labels"].astype(np.int32)
+    for r in (0, 1, 2, 5):
+        assert np.array_equal(handle.run(x, info["eps"], info["minpts"], r=r), ref)
+    with pytest.raises(dclust.DclustError) as e:
+        handle.run(x, info["eps"], info["minpts"], r=200)  # extent / 200 < eps
+    assert e.value.code == dclust.E_BINWIDTH
