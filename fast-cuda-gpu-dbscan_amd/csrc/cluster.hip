@@ -455,22 +455,37 @@ __device__ __forceinline__ int cell_offset_last(const Dev& d, const Inst& I, int
   return d.cell_z[B] - d.cell_z[A];
 }
 
+// Near lines above the cell (every cell of them has a larger index): line
+// (0,0) (cells one step up the line only), (1,0), (0,1), (1,-1), (1,1) in
+// 3-D; (0) and (1) in 2-D.  Nibble i of the constant is the i-th such row.
+template <int D>
+struct Upper {
+  static constexpr int lines = D == 3 ? 5 : 2;
+  static constexpr unsigned long long rows = D == 3 ? 0x87420ull : 0x20ull;
+  // nibble r: the mirror of near row r (the line with negated offsets)
+  static constexpr unsigned long long mirror = D == 3 ? 0x567834120ull : 0x120ull;
+};
+
 // Near adjacency of light cells: one lane per core point p of a light cell
-// (compacted list written by k_core).  The lanes of one cell examine the
-// near lines in a rotated order (lane rank r0 within the cell's run of
-// lanes starts at line r0) and share what they found through LDS after each
-// step, so a near cell found adjacent by one lane is not tested again by the
-// others.  A near cell is skipped when it is heavy (the heavy-cell pass sets
-// the bit), is already known adjacent, or its box lies beyond eps of p; it
-// is adjacent without a test when its whole box lies inside the guard band
-// around p and it holds a core point; otherwise its core points are tested
-// until one is joined to p.  The box distances use the same fixed-point
-// arithmetic as the distance, so they bound it.
+// (compacted list written by k_core).  Adjacency is symmetric (edge_pair
+// is), so a pair of light cells is decided once, by the lower cell: its
+// lanes examine only the near lines above it (Upper) and the bits they find
+// are mirrored into the upper cell's mask at the end.  The lanes of one
+// cell examine those lines in a rotated order (lane rank r0 within the
+// cell's run of lanes starts at line r0) and share what they found through
+// LDS after each step, so a near cell found adjacent by one lane is not
+// tested again by the others.  A near cell is skipped when it is heavy (the
+// heavy-cell pass sets the bits of both cells), is already known adjacent,
+// or its box lies beyond eps of p; it is adjacent without a test when its
+// whole box lies inside the guard band around p and it holds a core point;
+// otherwise its core points are tested until one is joined to p.  The box
+// distances use the same fixed-point arithmetic as the distance, so they
+// bound it.
 template <int D, bool F32>
 __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, int32_t c,
-                         bool act, int head, int rank, unsigned long long* xm,
+                         bool act, int head, int rank, int nrun, unsigned long long* xm,
                          unsigned long long& m0, Tally& t, long long& freeb) {
-  constexpr int NN = Near<D>::lines;
+  constexpr int NN = Upper<D>::lines;
   constexpr int B = 4;  // near cells need few tests: a short batch keeps registers low
   constexpr uint32_t kFrac = (1u << kFixBits) - 1u;
   uint32_t pg[D];
@@ -485,13 +500,16 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
   const int32_t zA = act ? d.cell_z[c] : 0;
   const RowsOf rows = rows_of(d, c);
   // the next line's packed row is loaded while this line is examined
-  int32_t pk_next = act ? rows[rank % NN] : 0;
+  auto row_at = [](int i) { return (int)((Upper<D>::rows >> (4 * i)) & 15ull); };
+  int32_t pk_next = act ? rows[row_at(rank % NN)] : 0;
   for (int step = 0; step < NN; ++step) {
-    const int r = (step + rank) % NN;
+    const int r = row_at((step + rank) % NN);
     const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
     const int o1 = D == 3 ? kRowOff3[r][1] : 0;
     const int32_t pk = pk_next;
-    if (step + 1 < NN) pk_next = act ? rows[(step + 1 + rank) % NN] : 0;
+    if (step + 1 < NN) pk_next = act ? rows[row_at((step + 1 + rank) % NN)] : 0;
+    // own line: only the cell one step up; other lines: offsets -1..1
+    const int ol_lo = r == 0 ? 1 : -1;
     const int32_t len = pk >> kRowCntShift;
     if (len > 0) {
       ++t.lines;
@@ -525,8 +543,8 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
         const float e = cell_gap(ol, fr[D - 1]), fe = cell_far(ol, fr[D - 1]);
         const bool reach = !(__builtin_fmaf(e, e, g2) > P.d2_out);
         const bool inside = __builtin_fmaf(fe, fe, h2) < P.d2_in;
-        bool use = k < len && ol >= -1 && ol <= 1 && nco[k] > 0 && n <= kHeavy && st + k != c &&
-                   reach && !((m0 >> (base + k)) & 1ull);
+        bool use = k < len && ol >= ol_lo && ol <= 1 && nco[k] > 0 && n <= kHeavy && reach &&
+                   !((m0 >> (base + k)) & 1ull);
         if (use && inside) {
           // every point of the cell is joined to p, and one of them is core
           m0 |= 1ull << (base + k);
@@ -575,6 +593,37 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
     wave_sync();
     if (act) m0 |= xm[head];
   }
+  // m0 is now the cell's whole find (this wave's run of its lanes).  Each
+  // bit (line r, cell k of its run) names an upper cell B; A's bit in B's
+  // mask is in B's mirrored line rm, at A's place in that line's run.  The
+  // bits are dealt out over the run's lanes, four at a time per lane so
+  // that their loads overlap.
+  if (!act) return;
+  unsigned long long u = m0;
+  int j = 0;
+  while (u) {
+    int32_t tb[4], rm[4], pos[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      while (u && j % nrun != rank) {
+        u &= u - 1;
+        ++j;
+      }
+      const int b = u ? __ffsll((long long)u) - 1 : -1;
+      if (u) {
+        u &= u - 1;
+        ++j;
+      }
+      const int r = b >= 0 ? (b * 205) >> 10 : 0;  // b / 5 for 0 <= b < 128
+      rm[q] = (int)((Upper<D>::mirror >> (4 * r)) & 15ull);
+      tb[q] = b >= 0 ? (rows[r] & kRowCellMask) + (b - 5 * r) : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pos[q] = tb[q] >= 0 ? rows_of(d, tb[q])[rm[q]] & kRowCellMask : 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (tb[q] >= 0) atomicOr(&d.amask[tb[q]], 1ull << (5 * rm[q] + (c - pos[q])));
+  }
 }
 
 template <bool F32>
@@ -595,6 +644,10 @@ __global__ __launch_bounds__(256, 5) void k_adj(Dev d) {
   const unsigned long long below = lane == 63 ? mh : (mh & ((2ull << lane) - 1ull));
   const int head = below ? 63 - __clzll((long long)below) : 0;
   const int rank = lane - head;
+  // run length: up to the next head or the end of the active lanes (a prefix)
+  const unsigned long long above = head == 63 ? 0ull : (mh & (~0ull << (head + 1)));
+  const int nact = __popcll(ballot(act));
+  const int nrun = (above ? __ffsll((long long)above) - 1 : nact) - head;
   s_m[wv][lane] = 0ull;
   wave_sync();
   int dim = 0;
@@ -610,8 +663,10 @@ __global__ __launch_bounds__(256, 5) void k_adj(Dev d) {
   // every lane runs the line loop (inactive lanes only take part in the
   // exchange), so the exchange points are uniform over the wave
   const bool any3 = ballot(act && dim == 3) != 0, any2 = ballot(act && dim == 2) != 0;
-  if (any3) adj_lane<3, F32>(d, *I, P, p, c, act && dim == 3, head, rank, s_m[wv], m0, t, freeb);
-  if (any2) adj_lane<2, F32>(d, *I, P, p, c, act && dim == 2, head, rank, s_m[wv], m0, t, freeb);
+  if (any3)
+    adj_lane<3, F32>(d, *I, P, p, c, act && dim == 3, head, rank, nrun, s_m[wv], m0, t, freeb);
+  if (any2)
+    adj_lane<2, F32>(d, *I, P, p, c, act && dim == 2, head, rank, nrun, s_m[wv], m0, t, freeb);
   // one atomic per cell run: the exchange slot of its head lane holds the
   // union of the run's finds
   if (act && lane == head) {
@@ -813,10 +868,45 @@ __global__ __launch_bounds__(1024) void k_cell_comp(Dev d) {
 // is taken by its lower cell) with core points in another component is
 // decided; a joined pair unions the two components (lock-free union-find on
 // the chain forest).
+// Stencil lines whose cells can lie above a cell (offsets (a, b) >= (0, 0)
+// in the key order, the own line included), in row order; -1 pads.
+#define DCLUST_UPPER_ROWS3 {0, 2, 4, 7, 8, 10, 12, 15, 16, 18, 20, 23, 24, -1, -1}
+#define DCLUST_UPPER_ROWS2 {0, 2, 4, -1, -1}
+static __constant__ int8_t kUpperRows3[15] = DCLUST_UPPER_ROWS3;
+static __constant__ int8_t kUpperRows2[5] = DCLUST_UPPER_ROWS2;
+namespace tables {
+constexpr int8_t up3[15] = DCLUST_UPPER_ROWS3;
+constexpr int8_t up2[5] = DCLUST_UPPER_ROWS2;
+constexpr bool above(int a, int b) { return a > 0 || (a == 0 && b >= 0); }
+constexpr bool upper_ok() {
+  int i = 0;
+  for (int r = 0; r < 25; ++r)
+    if (above(off3[r][0], off3[r][1]) && up3[i++] != r) return false;
+  if (i != 13 || up3[13] != -1 || up3[14] != -1) return false;
+  i = 0;
+  for (int r = 0; r < 5; ++r)
+    if (off2[r] >= 0 && up2[i++] != r) return false;
+  if (i != 3) return false;
+  // near rows of the adjacency pass (nibbles) and their mirrors
+  for (int k = 0; k < 5; ++k) {
+    const int r = (int)((Upper<3>::rows >> (4 * k)) & 15ull);
+    if (r > 8 || !above(off3[r][0], off3[r][1]) || up3[k] != r) return false;
+  }
+  for (int r = 0; r < 9; ++r)
+    if ((int)((Upper<3>::mirror >> (4 * r)) & 15ull) != mir3[r]) return false;
+  for (int k = 0; k < 2; ++k)
+    if ((int)((Upper<2>::rows >> (4 * k)) & 15ull) != up2[k]) return false;
+  for (int r = 0; r < 3; ++r)
+    if ((int)((Upper<2>::mirror >> (4 * r)) & 15ull) != mir2[r]) return false;
+  return true;
+}
+static_assert(upper_ok(), "upper stencil lines");
+}  // namespace tables
+
 template <int D, bool F32>
 __device__ long long far_cell(const Dev& d, const Inst& I, const Par& P, int32_t A, Tally& t,
                               long long& freeb) {
-  constexpr int NR = D == 3 ? 25 : 5;
+  constexpr int NR = D == 3 ? 13 : 3;  // upper lines
   const int32_t qs = d.cell_start[A], qe = d.cell_start[A + 1];
   const bool allc = d.cell_ncore[A] == qe - qs;
   const int32_t compA = d.cell_comp[A];
@@ -829,8 +919,12 @@ __device__ long long far_cell(const Dev& d, const Inst& I, const Par& P, int32_t
   // -1 for a cell without core points)
   for (int g = 0; g < NR; g += 5) {
     int32_t pk[5];
+    int rr[5];
 #pragma unroll
-    for (int j = 0; j < 5; ++j) pk[j] = rows[g + j];
+    for (int j = 0; j < 5; ++j) {
+      rr[j] = D == 3 ? kUpperRows3[g + j] : kUpperRows2[g + j];
+      pk[j] = rr[j] >= 0 ? rows[rr[j]] : 0;
+    }
     unsigned todo = 0u;
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
@@ -846,10 +940,13 @@ __device__ long long far_cell(const Dev& d, const Inst& I, const Par& P, int32_t
       todo &= todo - 1;
       const int j = (b * 205) >> 10;  // b / 5 for b < 25
       const int k = b - 5 * j;
-      const int r = g + j;
       int32_t pkj = pk[0];
+      int r = rr[0];
 #pragma unroll
-      for (int q = 1; q < 5; ++q) pkj = j == q ? pk[q] : pkj;
+      for (int q = 1; q < 5; ++q) {
+        pkj = j == q ? pk[q] : pkj;
+        r = j == q ? rr[q] : r;
+      }
       const int32_t B = (pkj & kRowCellMask) + k;
       const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
       const int o1 = D == 3 ? kRowOff3[r][1] : 0;
