@@ -661,7 +661,9 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     d.seg_pos = M.get<int64_t>(S_SEGPOS, shape.waves);
     d.q[0] = M.get<int32_t>(S_Q0, ntot);
     d.q[1] = M.get<int32_t>(S_Q1, ntot);
-    d.keyv = M.get<int32_t>(S_KEYV, ntot);
+    // keyv also holds the row pass's per-(tile, line group) marks first
+    d.keyv = M.get<int32_t>(S_KEYV, std::max<int64_t>(ntot, (ntot + 63) / 64 * 5));
+    d.rows_fb = d.keyv;
     d.flag = M.get<uint8_t>(S_FLAG, ntot);
     d.rank = M.get<int32_t>(S_RANK, ntot + 1);
     d.scan_tmp = M.get<int64_t>(S_SCAN, scan_tmp_entries(ntot));

@@ -686,8 +686,11 @@ __global__ __launch_bounds__(256, 5) void k_adj(Dev d) {
 // near cell with core points is decided in turn: its core points within
 // eps of the queries' bounding box are the candidates (one per lane), and
 // the queries near the candidates' bounding box are tried until some lane
-// finds a joined pair.  A joined light neighbour also gets the mirrored bit
-// (A lies in its mirrored stencil line).
+// finds a joined pair.  Light neighbours never test a heavy cell, so A
+// decides every light neighbour; a pair of heavy cells is decided once, by
+// the lower cell.  A joined neighbour also gets the mirrored bit (A lies in
+// its mirrored stencil line), and A's own bits are OR-ed into its mask
+// (lower heavy neighbours set bits there too).
 template <int D, bool F32>
 __device__ void adj_heavy(const Dev& d, const Inst& I, const Par& P, int32_t A, int32_t* s_qi,
                           uint32_t* s_qg, double* s_qx, Tally& t) {
@@ -746,6 +749,7 @@ __device__ void adj_heavy(const Dev& d, const Inst& I, const Par& P, int32_t A, 
       if (ol < -1 || ol > 1) continue;  // a far cell
       const int32_t b0 = d.cell_start[Bc], b1 = d.cell_start[Bc + 1];
       if (d.cell_ncore[Bc] == 0) continue;
+      if (Bc < A && b1 - b0 > kHeavy) continue;  // decided by the lower heavy cell
       bool edge = false;
       for (int32_t c0 = b0; c0 < b1 && !edge; c0 += 64) {
         const int32_t q = c0 + lane;
@@ -791,16 +795,14 @@ __device__ void adj_heavy(const Dev& d, const Inst& I, const Par& P, int32_t A, 
       }
       if (edge) {
         m0 |= 1ull << (5 * r + k);
-        if (b1 - b0 <= kHeavy) {
-          const int rm = D == 3 ? kRowMirror3[r] : kRowMirror2[r];
-          const int32_t pm = rows_of(d, Bc)[rm];
-          const int bm = 5 * rm + (A - (pm & kRowCellMask));
-          if (lane == 0) atomicOr(&d.amask[Bc], 1ull << bm);
-        }
+        const int rm = D == 3 ? kRowMirror3[r] : kRowMirror2[r];
+        const int32_t pm = rows_of(d, Bc)[rm];
+        const int bm = 5 * rm + (A - (pm & kRowCellMask));
+        if (lane == 0) atomicOr(&d.amask[Bc], 1ull << bm);
       }
     }
   }
-  if (lane == 0) d.amask[A] = m0;
+  if (lane == 0 && m0) atomicOr(&d.amask[A], m0);
 }
 
 template <bool F32>

@@ -672,12 +672,6 @@ __global__ __launch_bounds__(256) void k_line_mark(Dev d) {
   }
 }
 
-// For each (cell, stencil line): the run of non-empty cells of that line
-// whose last coordinate is within +-2 of the cell's, packed as
-// first | count << 28 (count <= 5).  The neighbour line's cell range comes
-// from the dense line table, or from two binary searches over all cell keys
-// when the table would be too large; inside the line the cells are found
-// by their last coordinate (cell_z), so the searches compare 32-bit values.
 // Five lower bounds lo[j] = first index in [lo[j], hi[j]) with a[i] >= v[j],
 // in lockstep (the loads of the five searches overlap).
 __device__ __forceinline__ void lower_bound5(const uint64_t* a, int32_t (&lo)[5],
@@ -702,57 +696,243 @@ __device__ __forceinline__ void lower_bound5(const uint64_t* a, int32_t (&lo)[5]
   }
 }
 
-__global__ __launch_bounds__(256) void k_rows(Dev d) {
+// Per-lane view of one cell and one line group: the cell's coordinates,
+// its window [zlo, zhi] along the last axis, its own line (global
+// line-table index), the shift to the neighbour lines of the group, and
+// which of the five neighbour lines exist.  3-D: the lines with offset
+// g - 2 along axis 0 and -2..2 along axis 1 (consecutive line-table
+// entries); 2-D: the five lines (g = 0).
+struct RowLane {
+  const Inst* I;
+  int32_t iid;
+  int dim;
+  int32_t zlo, zhi;
+  int64_t own, shift;
+  bool ok[5];
+};
+
+__device__ __forceinline__ void row_lane(const Dev& d, int32_t c, bool act, int g, int32_t C,
+                                         RowLane& L) {
+  const int32_t cc = act ? c : C - 1;
+  L.iid = d.ninst == 1 ? 0 : d.cell_inst[cc];
+  L.I = d.inst + L.iid;
+  const Inst& I = *L.I;
+  const int dim = I.dim;
+  L.dim = dim;
+  const int last = dim - 1;
+  int32_t kc[3];
+  unpack_cell(dim, d.cell_xyz[cc], kc);
+  L.zlo = kc[last] >= 2 ? kc[last] - 2 : 0;
+  L.zhi = kc[last] + 2 < I.nk[last] ? kc[last] + 2 : I.nk[last] - 1;
+  const int a = dim == 3 ? g - 2 : 0;
+  L.own = I.line_base + (dim == 3 ? (int64_t)kc[0] * I.nk[1] + kc[1] : (int64_t)kc[0]);
+  L.shift = dim == 3 ? (int64_t)a * I.nk[1] : 0;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    int o0 = a, o1 = j - 2;
+    if (dim != 3) {
+      o0 = kRowOff2[j];
+      o1 = 0;
+    }
+    const int32_t r0 = kc[0] + o0, r1 = kc[1] + o1;
+    bool ok = act && (dim == 3 || g == 0) && r0 >= 0 && r0 < I.nk[0];
+    if (dim == 3) ok = ok && r1 >= 0 && r1 < I.nk[1];
+    L.ok[j] = ok;
+  }
+}
+
+__device__ __forceinline__ int row_off(int dim, int j) { return dim == 3 ? j - 2 : kRowOff2[j]; }
+
+__device__ __forceinline__ void write_rows(const Dev& d, int32_t c, bool act, int dim, int g,
+                                           const int32_t* packed, long long& cand,
+                                           long long& nbrc) {
+  if (d.nbr_stats) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+      if (packed[j]) {
+        const int32_t a = packed[j] & kRowCellMask, b = a + (packed[j] >> kRowCntShift);
+        nbrc += b - a;
+        cand += (long long)(d.cell_start[c + 1] - d.cell_start[c]) *
+                (d.cell_start[b] - d.cell_start[a]);
+      }
+  }
+  if (act) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      // (2-D: groups g > 0 write the unused rows 5g .. 5g+4, empty)
+      const int r = dim == 3 ? kRowOfOff3[g][j] : 5 * g + j;
+      d.rows[(int64_t)r * d.rstride + c] = packed[j];
+    }
+  }
+}
+
+// For each (cell, stencil line): the run of non-empty cells of that line
+// whose last coordinate is within +-2 of the cell's, packed as
+// first | count << 28 (count <= 5).  One wavefront per tile of 64
+// consecutive cells and line group g.  The tile's cells lie in a few
+// consecutive lines (cells are sorted, so the first and last lanes bound
+// them), and so do their neighbour lines.  The wave copies that span of the
+// line table, then the last coordinates (cell_z) of the cells those lines
+// hold, into LDS with coalesced loads; every window is then found by a
+// branchless binary search in LDS of a depth uniform over the wave.  A tile
+// whose span does not fit the LDS slots (one that crosses an axis-0
+// boundary, long or sparse lines, two instances of a batch, no line table,
+// 2-D) is marked for k_rows_global.
+constexpr int RW_LINES = 256;  // line-table entries staged per wavefront
+constexpr int RW_CELLS = 512;  // last coordinates staged per wavefront
+
+__global__ __launch_bounds__(256, 8) void k_rows(Dev d) {
+  if (abandoned(d)) return;
+  __shared__ int2 s_ln[4][RW_LINES];
+  __shared__ int32_t s_cz[4][RW_CELLS];
+  const int32_t C = (int32_t)d.ctl[C_NCELL];
+  const int64_t NL = d.ctl[C_NLINES];
+  const bool table = NL <= d.line_cap;
+  const int g = blockIdx.y;
+  const int wv = threadIdx.x >> 6, lane = lane_id();
+  int2* ln = s_ln[wv];
+  int32_t* cz = s_cz[wv];
+  long long cand = 0, nbrc = 0;
+  const int64_t ntile = ((int64_t)C + 63) >> 6;
+  for (int64_t tile = (int64_t)blockIdx.x * 4 + wv; tile < ntile;
+       tile += (int64_t)gridDim.x * 4) {
+    const int32_t c = (int32_t)(tile * 64 + lane);
+    const bool act = c < C;
+    RowLane L;
+    row_lane(d, c, act, g, C, L);
+    bool anyl = false;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) anyl = anyl || L.ok[j];
+    if (!ballot(anyl)) {
+      // no neighbour line in the group for any lane (2-D groups g > 0, the
+      // grid's edge): empty rows
+      const int32_t zero[5] = {0, 0, 0, 0, 0};
+      if (lane == 0) d.rows_fb[tile * 5 + g] = 0;
+      write_rows(d, c, act, L.dim, g, zero, cand, nbrc);
+      continue;
+    }
+    // the tile's span of neighbour lines (wave-uniform)
+    const int nact = __popcll(ballot(act));
+    const int64_t own0 = rl_l(L.own, 0), own1 = rl_l(L.own, nact - 1);
+    const int i0 = rfl_i(L.iid), i1 = rl_i(L.iid, nact - 1);
+    const int64_t sh0 = rl_l(L.shift, 0);
+    int64_t lmin = own0 + sh0 - 2, lmax = own1 + sh0 + 2;
+    lmin = lmin < 0 ? 0 : lmin;
+    lmax = lmax >= NL ? NL - 1 : lmax;
+    // (2-D lines are long: five of them rarely fit the cell slots)
+    bool staged = table && i0 == i1 && rfl_i(L.dim) == 3 && lmax - lmin < RW_LINES;
+    wave_sync();  // the previous tile's LDS reads are done
+    int32_t cmin = 0;
+    int top = 0;
+    if (staged) {
+      const int span = lmax >= lmin ? (int)(lmax - lmin + 1) : 0;
+      int32_t mn = INT32_MAX, mx = -1, len = 0;
+#pragma unroll
+      for (int k = 0; k < RW_LINES / 64; ++k) {
+        const int i = lane + 64 * k;
+        const int2 e = i < span ? d.lines[lmin + i] : make_int2(-1, -1);
+        if (i < span) ln[i] = e;
+        if (e.x >= 0 && e.y > e.x) {
+          mn = e.x < mn ? e.x : mn;
+          mx = e.y > mx ? e.y : mx;
+          len = e.y - e.x > len ? e.y - e.x : len;
+        }
+      }
+      // lines are in cell order: the staged cells are [first start, last end)
+      cmin = wave_min(mn);
+      const int32_t cmax = wave_max(mx);
+      len = wave_max(len);
+      const int ncz = cmax > cmin ? cmax - cmin : 0;
+      staged = ncz <= RW_CELLS;
+      if (staged) {
+        for (int i = lane; i < ncz; i += 64) cz[i] = d.cell_z[cmin + i];
+        top = len > 0 ? 1 << (31 - __clz(len)) : 0;  // highest power of 2 <= len
+      }
+      wave_sync();
+    }
+    if (lane == 0) d.rows_fb[tile * 5 + g] = staged ? 0 : 1;
+    if (!staged) continue;
+    int32_t base[5], n[5], pos[5], ls[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const int2 e = L.ok[j] ? ln[(int)(L.own + L.shift + row_off(L.dim, j) - lmin)]
+                             : make_int2(-1, -1);
+      const bool live = e.x >= 0 && e.y > e.x;
+      ls[j] = e.x;
+      n[j] = live ? e.y - e.x : 0;
+      base[j] = live ? e.x - cmin : 0;
+      pos[j] = -1;  // last index with z < zlo
+    }
+    for (int h = top; h > 0; h >>= 1) {
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        const int32_t q = pos[j] + h;
+        const int32_t v = cz[base[j] + (q < n[j] ? q : 0)];
+        pos[j] = (q < n[j] && v < L.zlo) ? q : pos[j];
+      }
+    }
+    int32_t packed[5];
+    // the run ends at the last index with z <= zhi, at most five after pos
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      int32_t e = pos[j];
+#pragma unroll
+      for (int h = 4; h > 0; h >>= 1) {
+        const int32_t q = e + h;
+        const int32_t v = cz[base[j] + (q < n[j] ? q : 0)];
+        e = (q < n[j] && v <= L.zhi) ? q : e;
+      }
+      const int32_t cnt = e - pos[j];
+      packed[j] = (n[j] > 0 && cnt > 0) ? ((ls[j] + pos[j] + 1) | (cnt << kRowCntShift)) : 0;
+    }
+    write_rows(d, c, act, L.dim, g, packed, cand, nbrc);
+  }
+  stat_add(d, C_CAND, cand);
+  stat_add(d, C_NBRC, nbrc);
+}
+
+// The (tile, line group) pairs k_rows left: one lane per cell, the line's
+// cell range from the line table (or by two lower bounds over all cell
+// keys when there is none), then the first cell with z >= zlo by binary
+// search over cell_z in global memory, the five lines' searches in lockstep
+// so that their loads overlap.
+__global__ __launch_bounds__(256) void k_rows_global(Dev d) {
   if (abandoned(d)) return;
   const int32_t C = (int32_t)d.ctl[C_NCELL];
   const bool table = d.ctl[C_NLINES] <= d.line_cap;
-  // 3-D: the five lines with offset g - 2 along axis 0 (consecutive entries
-  // of the line table); 2-D: the five lines (g = 0)
-  const int g = blockIdx.y;
+  const int wv = threadIdx.x >> 6, lane = lane_id();
   long long cand = 0, nbrc = 0;
-  // one lane per cell, consecutive cells in consecutive lanes: they mostly
-  // share their neighbour lines (broadcast loads); each lane has five
-  // lines' loads in flight
-  for (int32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x) {
-    const Inst& I = *inst_of_cell(d, c);
-    const int dim = I.dim;
-    const int last = dim - 1;
-    int32_t kc[3];
-    unpack_cell(dim, d.cell_xyz[c], kc);
-    const int32_t zlo = kc[last] >= 2 ? kc[last] - 2 : 0;
-    const int32_t zhi = kc[last] + 2 < I.nk[last] ? kc[last] + 2 : I.nk[last] - 1;
+  const int64_t items = (((int64_t)C + 63) >> 6) * 5;
+  for (int64_t it = (int64_t)blockIdx.x * 4 + wv; it < items; it += (int64_t)gridDim.x * 4) {
+    if (!d.rows_fb[it]) continue;
+    const int64_t tile = it / 5;
+    const int g = (int)(it - tile * 5);
+    const int32_t c = (int32_t)(tile * 64 + lane);
+    const bool act = c < C;
+    RowLane L;
+    row_lane(d, c, act, g, C, L);
+    const Inst& I = *L.I;
+    const int last = L.dim - 1;
     int32_t ls[5], le[5];
     uint64_t k0[5];  // first key of each neighbour line (no line table)
     bool need = false;
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
-      int o0 = 0, o1 = 0;
-      if (dim == 3) {
-        o0 = g - 2;
-        o1 = j - 2;
-      } else {
-        o0 = kRowOff2[j];
-      }
-      const int32_t r0 = kc[0] + o0, r1 = kc[1] + o1;
-      bool ok = (dim == 3 || g == 0) && r0 >= 0 && r0 < I.nk[0];
-      if (dim == 3) ok = ok && r1 >= 0 && r1 < I.nk[1];
-      const int64_t line = dim == 3 ? (int64_t)r0 * I.nk[1] + r1 : (int64_t)r0;
+      const int64_t line = L.own + L.shift + row_off(L.dim, j) - I.line_base;
       ls[j] = -1;
       le[j] = -1;
       k0[j] = (uint64_t)(I.key_base + line * I.nk[last]);
-      if (ok && table) {
+      if (L.ok[j] && table) {
         const int2 lr = d.lines[I.line_base + line];
         ls[j] = lr.x;
         le[j] = lr.y;
-      } else if (ok) {
+      } else if (L.ok[j]) {
         ls[j] = 0;  // searched below
         le[j] = C;
         need = true;
       }
     }
     if (need) {
-      // no line table: the line's cell range by two lower bounds over all
-      // cell keys, the five lines' searches in lockstep
       int32_t hi[5];
 #pragma unroll
       for (int j = 0; j < 5; ++j) hi[j] = le[j];
@@ -768,8 +948,7 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
       for (int j = 0; j < 5; ++j)
         if (le[j] <= ls[j]) ls[j] = le[j] = -1;  // empty line
     }
-    // lines of more than 4 cells: first cell with z >= zlo, the five
-    // searches in lockstep so that their loads overlap
+    // lines of more than 4 cells: first cell with z >= zlo
     int32_t lo[5], hi[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
@@ -790,7 +969,7 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
       for (int j = 0; j < 5; ++j)
         if (lo[j] < hi[j]) {
           const int32_t mid = (lo[j] + hi[j]) >> 1;
-          if (zm[j] < zlo)
+          if (zm[j] < L.zlo)
             lo[j] = mid + 1;
           else
             hi[j] = mid;
@@ -798,40 +977,23 @@ __global__ __launch_bounds__(256) void k_rows(Dev d) {
     }
     // from the first candidate (line start of a short line, search result
     // of a long one) the next up to 5 cells decide the run
-    int32_t zz[5][5];
-    int32_t a0[5];
+    int32_t packed[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
       const bool live = ls[j] >= 0 && le[j] > ls[j];
       const bool shortl = le[j] - ls[j] <= 4;
-      a0[j] = shortl ? ls[j] : lo[j];
-#pragma unroll
-      for (int i = 0; i < 5; ++i)
-        zz[j][i] = (live && a0[j] + i < le[j] && (i < 4 || !shortl)) ? d.cell_z[a0[j] + i]
-                                                                     : 0x7fffffff;
-    }
-    int32_t packed[5];
-#pragma unroll
-    for (int j = 0; j < 5; ++j) {
-      int32_t a = a0[j], b = a0[j];
+      const int32_t a0 = shortl ? ls[j] : lo[j];
+      int32_t a = a0, b = a0;
 #pragma unroll
       for (int i = 0; i < 5; ++i) {
-        a += zz[j][i] < zlo;
-        b += zz[j][i] <= zhi;
+        const int32_t z = (live && a0 + i < le[j] && (i < 4 || !shortl)) ? d.cell_z[a0 + i]
+                                                                         : 0x7fffffff;
+        a += z < L.zlo;
+        b += z <= L.zhi;
       }
       packed[j] = (ls[j] >= 0 && b > a) ? (a | ((b - a) << kRowCntShift)) : 0;
-      if (d.nbr_stats && packed[j]) {
-        nbrc += b - a;
-        cand += (long long)(d.cell_start[c + 1] - d.cell_start[c]) *
-                (d.cell_start[b] - d.cell_start[a]);
-      }
     }
-#pragma unroll
-    for (int j = 0; j < 5; ++j) {
-      // (2-D: groups g > 0 write the unused rows 5g .. 5g+4, empty)
-      const int r = dim == 3 ? kRowOfOff3[g][j] : 5 * g + j;
-      d.rows[(int64_t)r * d.rstride + c] = packed[j];
-    }
+    write_rows(d, c, act, L.dim, g, packed, cand, nbrc);
   }
   stat_add(d, C_CAND, cand);
   stat_add(d, C_NBRC, nbrc);
@@ -879,6 +1041,7 @@ void launch_rows(const Dev& d, int64_t n, int grid_cells, hipStream_t s) {
   const int64_t gb = (n + 255) / 256;
   const int rb = (int)std::max<int64_t>(1, std::min<int64_t>(4LL * grid_cells, gb));
   hipLaunchKernelGGL(k_rows, dim3(rb, 5), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_rows_global, dim3(rb), dim3(256), 0, s, d);
 }
 
 void launch_scan_top(int64_t* v, int nb, hipStream_t s) {

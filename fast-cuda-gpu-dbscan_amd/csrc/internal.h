@@ -259,6 +259,10 @@ struct Dev {
   int32_t* q[2];
   int64_t* seg_pos;     // [waves] seeding cursor of each expansion wave
   int32_t* keyv;
+  // per (tile of 64 cells, line group): 1 = the rows are left to
+  // k_rows_global (the same memory as keyv, which is free until the
+  // output stage)
+  int32_t* rows_fb;
   uint8_t* flag;
   int32_t* rank;
   int64_t* scan_tmp;  // block sums for scans
@@ -294,6 +298,11 @@ __device__ __forceinline__ double rl_d(double v, int l) {
   int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
   int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
   return __longlong_as_double(((long long)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ int64_t rl_l(int64_t v, int l) {
+  const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 __device__ __forceinline__ int rfl_i(int v) {
   return __builtin_amdgcn_readfirstlane(v);
