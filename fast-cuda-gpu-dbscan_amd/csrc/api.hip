@@ -104,7 +104,7 @@ enum Slot {
   S_G0, S_G1, S_G2, S_ORIG, S_CORE, S_LABEL, S_PARENT, S_COMPMIN, S_PAIRS, S_Q0, S_Q1,
   S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT, S_PAIRCNT, S_SEGPOS, S_LINES, S_AMASK,
   S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP, S_ADJPTS, S_CORECELLS, S_TUNELAB, S_STAT, S_GLINES,
-  S_CZ, S_XOFF, S_OS_LABEL, S_OS_STATE, S_OS_SEEDS, S_OS_LEN, S_OS_CONTACT, S_OS_FIRST
+  S_CZ, S_XOFF, S_OS_LABEL, S_OS_STATE, S_OS_SEEDS, S_OS_LEN, S_OS_CONTACT, S_OS_FIRST, S_ROWSFB
 };
 
 // line table entries (8 bytes each) above which neighbour lines are found
@@ -634,6 +634,9 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
                     (unsigned long long*)(d.stat + (int64_t)C_NHEAVY * kStripes), cap_p};
     d.core_cells = SList{M.get<int32_t>(S_CORECELLS, (size_t)cap_c * kStripes),
                          (unsigned long long*)(d.stat + (int64_t)C_NCORECELLS * kStripes), cap_c};
+    const int64_t cap_r = row_list_cap(ntot);
+    d.rows_fb = SList{M.get<int32_t>(S_ROWSFB, (size_t)cap_r * kStripes),
+                      (unsigned long long*)(d.stat + (int64_t)C_NROWSFB * kStripes), cap_r};
     d.cell_inst = M.get<int32_t>(S_CINST, ntot);
     d.cell_ncore = M.get<int32_t>(S_CM, ntot);
     d.rows = M.get<int32_t>(S_ROWS, (size_t)ntot * kMaxRows);
@@ -661,9 +664,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     d.seg_pos = M.get<int64_t>(S_SEGPOS, shape.waves);
     d.q[0] = M.get<int32_t>(S_Q0, ntot);
     d.q[1] = M.get<int32_t>(S_Q1, ntot);
-    // keyv also holds the row pass's per-(tile, line group) marks first
-    d.keyv = M.get<int32_t>(S_KEYV, std::max<int64_t>(ntot, (ntot + 63) / 64 * 5));
-    d.rows_fb = d.keyv;
+    d.keyv = M.get<int32_t>(S_KEYV, ntot);
     d.flag = M.get<uint8_t>(S_FLAG, ntot);
     d.rank = M.get<int32_t>(S_RANK, ntot + 1);
     d.scan_tmp = M.get<int64_t>(S_SCAN, scan_tmp_entries(ntot));

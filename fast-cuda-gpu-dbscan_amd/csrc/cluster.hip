@@ -164,29 +164,6 @@ __device__ __forceinline__ void mask_set(unsigned long long& m0, unsigned long l
     }                                              \
   } while (0)
 
-// Appends `value` of the threads with `take` to the workgroup's sub-list of
-// L, in thread order, with one atomic per workgroup.  Every thread of the
-// workgroup calls it.
-__device__ __forceinline__ void block_append(bool take, int32_t value, const SList& L) {
-  __shared__ int s_cnt[16];
-  __shared__ long long s_base;
-  const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const unsigned long long m = ballot(take);
-  if (lane_id() == 0) s_cnt[wv] = __popcll(m);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int tot = 0;
-    for (int w = 0; w < nw; ++w) tot += s_cnt[w];
-    s_base = tot ? (long long)atomicAdd(&L.cnt[blockIdx.x & (kStripes - 1)],
-                                        (unsigned long long)tot)
-                 : 0;
-  }
-  __syncthreads();
-  long long base = s_base;
-  for (int w = 0; w < wv; ++w) base += s_cnt[w];
-  if (take) L.data[(int64_t)(blockIdx.x & (kStripes - 1)) * L.cap + base + mbcnt(m)] = value;
-  __syncthreads();  // s_cnt / s_base are reused by the next call
-}
 
 // ------------------------------------------------------------------ core --
 // Neighbour count of point p with early exit at minpts.  Every candidate

@@ -768,127 +768,148 @@ __device__ __forceinline__ void write_rows(const Dev& d, int32_t c, bool act, in
 
 // For each (cell, stencil line): the run of non-empty cells of that line
 // whose last coordinate is within +-2 of the cell's, packed as
-// first | count << 28 (count <= 5).  One wavefront per tile of 64
-// consecutive cells and line group g.  The tile's cells lie in a few
+// first | count << 28 (count <= 5).  One wavefront per (tile of 64
+// consecutive cells, line group g).  The tile's cells lie in a few
 // consecutive lines (cells are sorted, so the first and last lanes bound
-// them), and so do their neighbour lines.  The wave copies that span of the
-// line table, then the last coordinates (cell_z) of the cells those lines
-// hold, into LDS with coalesced loads; every window is then found by a
-// branchless binary search in LDS of a depth uniform over the wave.  A tile
-// whose span does not fit the LDS slots (one that crosses an axis-0
-// boundary, long or sparse lines, two instances of a batch, no line table,
-// 2-D) is marked for k_rows_global.
-constexpr int RW_LINES = 256;  // line-table entries staged per wavefront
-constexpr int RW_CELLS = 512;  // last coordinates staged per wavefront
+// them), and so do their neighbour lines: the line table gives the range
+// of cells those lines hold, and the wave copies the cells' sort keys,
+// relative to the span's first line (below 2^29), into LDS with coalesced
+// loads, padded with ~0 to a larger power of two.  A window is then two
+// searches in that sorted array: the first key >= the key of (line, zlo),
+// by a branchless binary search of a depth uniform over the wave, and the
+// last key <= the key of (line, zhi), at most five further.  A tile whose
+// cells do not fit the LDS slots (one that crosses an axis-0 boundary,
+// long or sparse lines, two instances of a batch, no line table, 2-D) is
+// listed for k_rows_global.
+constexpr int RW_LINES = 256;  // line-table entries of a tile's span (at most)
+constexpr int RW_CELLS = 512;  // cell keys staged per wavefront (a power of two)
+
+// Rows of one (tile, line group) from LDS; false when the tile does not fit
+// (then nothing is written).  3-D tiles of one instance only, so the
+// instance's fields are wave-uniform (scalar loads).
+__device__ __forceinline__ bool rows_staged(const Dev& d, int64_t tile, int g, int32_t C,
+                                            int64_t NL, uint32_t* ck) {
+  const int lane = lane_id();
+  const int32_t c = (int32_t)(tile * 64 + lane);
+  const bool act = c < C;
+  const int32_t cc = act ? c : C - 1;
+  const int nact = __popcll(ballot(act));
+  int32_t iid = 0;
+  if (d.ninst != 1) {
+    const int32_t v = d.cell_inst[cc];
+    iid = rfl_i(v);
+    if (iid != rl_i(v, nact - 1)) return false;
+  }
+  const Inst& I = d.inst[iid];
+  if (I.dim != 3) {
+    // 2-D: groups g > 0 hold no lines (their rows are written empty); the
+    // five lines of g = 0 are long and rarely fit the slots
+    if (g == 0) return false;
+    if (act) {
+#pragma unroll
+      for (int j = 0; j < 5; ++j) d.rows[(int64_t)(5 * g + j) * d.rstride + c] = 0;
+    }
+    return true;
+  }
+  const int32_t nk0 = I.nk[0], nk1 = I.nk[1], nk2 = I.nk[2];
+  const uint64_t pc = d.cell_xyz[cc];
+  const int32_t x = (int32_t)(pc & 0x1fffff), y = (int32_t)((pc >> 21) & 0x1fffff);
+  const int32_t z = (int32_t)(pc >> 42);
+  const int a = g - 2;
+  const bool okx = act & (x + a >= 0) & (x + a < nk0);
+  // (the lines of a tile's first and last lanes bound its lines: cells are
+  // sorted)
+  const int64_t own = (int64_t)x * nk1 + y;  // within the instance
+  const int64_t own0 = rl_l(own, 0), own1 = rl_l(own, nact - 1);
+  const int64_t nl = (int64_t)nk0 * nk1;
+  int64_t lmin = own0 + (int64_t)a * nk1 - 2, lmax = own1 + (int64_t)a * nk1 + 2;
+  lmin = lmin < 0 ? 0 : lmin;
+  lmax = lmax >= nl ? nl - 1 : lmax;
+  if (lmax - lmin >= RW_LINES) return false;
+  const int64_t gl = I.line_base + lmin;  // first line-table entry of the span
+  // the cells of the span: [first start, last end) of its non-empty lines
+  // (the lines are in cell order: the first and last non-empty entries)
+  const int span = lmax >= lmin ? (int)(lmax - lmin + 1) : 0;
+  int32_t cmin = -1, cmax = -1;
+#pragma unroll
+  for (int k = 0; k < RW_LINES / 64; ++k) {
+    if (64 * k < span) {
+      const int i = lane + 64 * k;
+      const int2 e = i < span && gl + i < NL ? d.lines[gl + i] : make_int2(-1, -1);
+      const unsigned long long m = ballot(e.x >= 0 && e.y > e.x);
+      if (m) {
+        if (cmin < 0) cmin = rl_i(e.x, __ffsll((long long)m) - 1);
+        cmax = rl_i(e.y, 63 - __clzll((long long)m));
+      }
+    }
+  }
+  const int ncz = cmin >= 0 ? cmax - cmin : 0;
+  if (ncz >= RW_CELLS) return false;
+  cmin = cmin < 0 ? 0 : cmin;
+  // a power of two > ncz: the last slot is always a ~0 sentinel, so the
+  // search (which reaches index pad - 2 at most) is exact
+  const int pad = 1 << (32 - __clz(ncz));
+  const uint64_t kbase = (uint64_t)(I.key_base + lmin * nk2);
+  for (int i = lane; i < pad; i += 64)
+    ck[i] = i < ncz ? (uint32_t)(d.cell_key[cmin + i] - kbase) : 0xffffffffu;
+  wave_sync();
+  // keys of (line of j, zlo), relative to the span's first line
+  const int32_t zlo = z >= 2 ? z - 2 : 0, zhi = z + 2 < nk2 ? z + 2 : nk2 - 1;
+  const uint32_t k0 = (uint32_t)((own + (int64_t)a * nk1 - 2 - lmin) * nk2) + (uint32_t)zlo;
+  uint32_t klo[5];
+  int32_t pos[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    klo[j] = k0 + (uint32_t)(j * nk2);
+    pos[j] = -1;  // last index with key < klo
+  }
+  for (int h = pad >> 1; h > 0; h >>= 1) {
+    uint32_t v[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) v[j] = ck[pos[j] + h];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) pos[j] = v[j] < klo[j] ? pos[j] + h : pos[j];
+  }
+  const uint32_t dz = (uint32_t)(zhi - zlo);
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    // the run: keys up to (line, zhi), at most five after pos
+    int32_t e = pos[j];
+#pragma unroll
+    for (int h = 4; h > 0; h >>= 1) {
+      const int32_t q = e + h;
+      e = (q < pad && ck[q < pad ? q : 0] <= klo[j] + dz) ? q : e;
+    }
+    const int32_t cnt = e - pos[j];
+    const bool ok = okx & (y + j - 2 >= 0) & (y + j - 2 < nk1);
+    const int32_t packed = (ok && cnt > 0) ? ((cmin + pos[j] + 1) | (cnt << kRowCntShift)) : 0;
+    if (act) d.rows[(int64_t)kRowOfOff3[g][j] * d.rstride + c] = packed;
+  }
+  return true;
+}
 
 __global__ __launch_bounds__(256, 8) void k_rows(Dev d) {
   if (abandoned(d)) return;
-  __shared__ int2 s_ln[4][RW_LINES];
-  __shared__ int32_t s_cz[4][RW_CELLS];
+  __shared__ uint32_t s_ck[4][RW_CELLS];
   const int32_t C = (int32_t)d.ctl[C_NCELL];
   const int64_t NL = d.ctl[C_NLINES];
   const bool table = NL <= d.line_cap;
-  const int g = blockIdx.y;
-  const int wv = threadIdx.x >> 6, lane = lane_id();
-  int2* ln = s_ln[wv];
-  int32_t* cz = s_cz[wv];
-  long long cand = 0, nbrc = 0;
-  const int64_t ntile = ((int64_t)C + 63) >> 6;
-  for (int64_t tile = (int64_t)blockIdx.x * 4 + wv; tile < ntile;
-       tile += (int64_t)gridDim.x * 4) {
-    const int32_t c = (int32_t)(tile * 64 + lane);
-    const bool act = c < C;
-    RowLane L;
-    row_lane(d, c, act, g, C, L);
-    bool anyl = false;
-#pragma unroll
-    for (int j = 0; j < 5; ++j) anyl = anyl || L.ok[j];
-    if (!ballot(anyl)) {
-      // no neighbour line in the group for any lane (2-D groups g > 0, the
-      // grid's edge): empty rows
-      const int32_t zero[5] = {0, 0, 0, 0, 0};
-      if (lane == 0) d.rows_fb[tile * 5 + g] = 0;
-      write_rows(d, c, act, L.dim, g, zero, cand, nbrc);
-      continue;
+  const int wv = threadIdx.x >> 6;
+  // work item: tile * 5 + line group; the four waves of a workgroup take
+  // four consecutive items per step (the steps are uniform over the
+  // workgroup, as the list append wants)
+  const int64_t items = (((int64_t)C + 63) >> 6) * 5;
+  for (int64_t base = (int64_t)blockIdx.x * 4; base < items; base += (int64_t)gridDim.x * 4) {
+    const int64_t it = base + wv;
+    bool left = false;
+    if (it < items) {
+      const int64_t tile = it / 5;
+      // (the neighbourhood statistics run takes the global-memory pass)
+      left = !(table && !d.nbr_stats &&
+               rows_staged(d, tile, (int)(it - tile * 5), C, NL, s_ck[wv]));
     }
-    // the tile's span of neighbour lines (wave-uniform)
-    const int nact = __popcll(ballot(act));
-    const int64_t own0 = rl_l(L.own, 0), own1 = rl_l(L.own, nact - 1);
-    const int i0 = rfl_i(L.iid), i1 = rl_i(L.iid, nact - 1);
-    const int64_t sh0 = rl_l(L.shift, 0);
-    int64_t lmin = own0 + sh0 - 2, lmax = own1 + sh0 + 2;
-    lmin = lmin < 0 ? 0 : lmin;
-    lmax = lmax >= NL ? NL - 1 : lmax;
-    // (2-D lines are long: five of them rarely fit the cell slots)
-    bool staged = table && i0 == i1 && rfl_i(L.dim) == 3 && lmax - lmin < RW_LINES;
-    wave_sync();  // the previous tile's LDS reads are done
-    int32_t cmin = 0;
-    int top = 0;
-    if (staged) {
-      const int span = lmax >= lmin ? (int)(lmax - lmin + 1) : 0;
-      int32_t mn = INT32_MAX, mx = -1, len = 0;
-#pragma unroll
-      for (int k = 0; k < RW_LINES / 64; ++k) {
-        const int i = lane + 64 * k;
-        const int2 e = i < span ? d.lines[lmin + i] : make_int2(-1, -1);
-        if (i < span) ln[i] = e;
-        if (e.x >= 0 && e.y > e.x) {
-          mn = e.x < mn ? e.x : mn;
-          mx = e.y > mx ? e.y : mx;
-          len = e.y - e.x > len ? e.y - e.x : len;
-        }
-      }
-      // lines are in cell order: the staged cells are [first start, last end)
-      cmin = wave_min(mn);
-      const int32_t cmax = wave_max(mx);
-      len = wave_max(len);
-      const int ncz = cmax > cmin ? cmax - cmin : 0;
-      staged = ncz <= RW_CELLS;
-      if (staged) {
-        for (int i = lane; i < ncz; i += 64) cz[i] = d.cell_z[cmin + i];
-        top = len > 0 ? 1 << (31 - __clz(len)) : 0;  // highest power of 2 <= len
-      }
-      wave_sync();
-    }
-    if (lane == 0) d.rows_fb[tile * 5 + g] = staged ? 0 : 1;
-    if (!staged) continue;
-    int32_t base[5], n[5], pos[5], ls[5];
-#pragma unroll
-    for (int j = 0; j < 5; ++j) {
-      const int2 e = L.ok[j] ? ln[(int)(L.own + L.shift + row_off(L.dim, j) - lmin)]
-                             : make_int2(-1, -1);
-      const bool live = e.x >= 0 && e.y > e.x;
-      ls[j] = e.x;
-      n[j] = live ? e.y - e.x : 0;
-      base[j] = live ? e.x - cmin : 0;
-      pos[j] = -1;  // last index with z < zlo
-    }
-    for (int h = top; h > 0; h >>= 1) {
-#pragma unroll
-      for (int j = 0; j < 5; ++j) {
-        const int32_t q = pos[j] + h;
-        const int32_t v = cz[base[j] + (q < n[j] ? q : 0)];
-        pos[j] = (q < n[j] && v < L.zlo) ? q : pos[j];
-      }
-    }
-    int32_t packed[5];
-    // the run ends at the last index with z <= zhi, at most five after pos
-#pragma unroll
-    for (int j = 0; j < 5; ++j) {
-      int32_t e = pos[j];
-#pragma unroll
-      for (int h = 4; h > 0; h >>= 1) {
-        const int32_t q = e + h;
-        const int32_t v = cz[base[j] + (q < n[j] ? q : 0)];
-        e = (q < n[j] && v <= L.zhi) ? q : e;
-      }
-      const int32_t cnt = e - pos[j];
-      packed[j] = (n[j] > 0 && cnt > 0) ? ((ls[j] + pos[j] + 1) | (cnt << kRowCntShift)) : 0;
-    }
-    write_rows(d, c, act, L.dim, g, packed, cand, nbrc);
+    block_append(lane_id() == 0 && left, (int32_t)it, d.rows_fb);
   }
-  stat_add(d, C_CAND, cand);
-  stat_add(d, C_NBRC, nbrc);
 }
 
 // The (tile, line group) pairs k_rows left: one lane per cell, the line's
@@ -898,13 +919,14 @@ __global__ __launch_bounds__(256, 8) void k_rows(Dev d) {
 // so that their loads overlap.
 __global__ __launch_bounds__(256) void k_rows_global(Dev d) {
   if (abandoned(d)) return;
+  __shared__ long long s_pre[kStripes + 1];
+  const long long nit = slist_prefix(d.rows_fb, s_pre);
   const int32_t C = (int32_t)d.ctl[C_NCELL];
   const bool table = d.ctl[C_NLINES] <= d.line_cap;
   const int wv = threadIdx.x >> 6, lane = lane_id();
   long long cand = 0, nbrc = 0;
-  const int64_t items = (((int64_t)C + 63) >> 6) * 5;
-  for (int64_t it = (int64_t)blockIdx.x * 4 + wv; it < items; it += (int64_t)gridDim.x * 4) {
-    if (!d.rows_fb[it]) continue;
+  for (long long w = (long long)blockIdx.x * 4 + wv; w < nit; w += (long long)gridDim.x * 4) {
+    const int64_t it = slist_at(d.rows_fb, s_pre, w);
     const int64_t tile = it / 5;
     const int g = (int)(it - tile * 5);
     const int32_t c = (int32_t)(tile * 64 + lane);
@@ -1035,13 +1057,24 @@ void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
   hipLaunchKernelGGL(k_line_mark, dim3(grid_cells), dim3(256), 0, s, d);
 }
 
+// Row pass grid: one wavefront per (tile of 64 cells, line group) for up
+// to n cells, at most kRowGrid workgroups (the steps loop beyond).
+constexpr int64_t kRowGrid = 16384;
+int64_t row_blocks(int64_t n) {
+  return std::max<int64_t>(1, std::min<int64_t>(kRowGrid, (((n + 63) / 64) * 5 + 3) / 4));
+}
+int64_t row_list_cap(int64_t n) {
+  const int64_t items = ((n + 63) / 64) * 5;
+  const int64_t g = row_blocks(n);
+  const int64_t steps = (items + 4 * g - 1) / (4 * g);
+  return (g + kStripes - 1) / kStripes * 4 * steps;  // appends per stripe, at most
+}
+
 void launch_rows(const Dev& d, int64_t n, int grid_cells, hipStream_t s) {
-  // 4 x grid_cells workgroups per line group, but no more than the cells
-  // (at most n) need: the dependent searches want many waves in flight
-  const int64_t gb = (n + 255) / 256;
-  const int rb = (int)std::max<int64_t>(1, std::min<int64_t>(4LL * grid_cells, gb));
-  hipLaunchKernelGGL(k_rows, dim3(rb, 5), dim3(256), 0, s, d);
-  hipLaunchKernelGGL(k_rows_global, dim3(rb), dim3(256), 0, s, d);
+  hipLaunchKernelGGL(k_rows, dim3((unsigned)row_blocks(n)), dim3(256), 0, s, d);
+  // the tiles left over, one wavefront each: enough waves for the list
+  const int64_t gb = std::max<int64_t>(1, std::min<int64_t>(4LL * grid_cells, (n + 255) / 256));
+  hipLaunchKernelGGL(k_rows_global, dim3((unsigned)gb), dim3(256), 0, s, d);
 }
 
 void launch_scan_top(int64_t* v, int nb, hipStream_t s) {

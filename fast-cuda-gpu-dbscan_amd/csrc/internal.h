@@ -154,9 +154,10 @@ enum Ctl : int {
   C_CORE_LINES,   // k_core: non-empty stencil lines streamed
   C_ADJ_LINES,    // k_adj: non-empty near lines examined
   C_ADJ_CELLS,    // k_adj: cells of those lines
+  C_NROWSFB,      // row work items left to the global-memory pass (Dev::rows_fb)
   C_NCTL = 48
 };
-static_assert(C_ADJ_CELLS < C_NCTL, "control block too small");
+static_assert(C_NROWSFB < C_NCTL, "control block too small");
 
 // Cell-key space: the sort key of a cell is its instance's key base plus its
 // linear index, and the radix sort orders keys of up to 63 bits, so the
@@ -259,10 +260,7 @@ struct Dev {
   int32_t* q[2];
   int64_t* seg_pos;     // [waves] seeding cursor of each expansion wave
   int32_t* keyv;
-  // per (tile of 64 cells, line group): 1 = the rows are left to
-  // k_rows_global (the same memory as keyv, which is free until the
-  // output stage)
-  int32_t* rows_fb;
+  SList rows_fb;         // row work items (tile * 5 + line group) left to k_rows_global
   uint8_t* flag;
   int32_t* rank;
   int64_t* scan_tmp;  // block sums for scans
@@ -370,6 +368,30 @@ __device__ __forceinline__ void stat_add(const Dev& d, int ctr, long long v) {
     const unsigned w = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kStripes - 1);
     atomicAdd((unsigned long long*)&d.stat[ctr * kStripes + w], (unsigned long long)s);
   }
+}
+
+// Appends `value` of the threads with `take` to the workgroup's sub-list of
+// L, in thread order, with one atomic per workgroup.  Every thread of the
+// workgroup calls it.
+__device__ __forceinline__ void block_append(bool take, int32_t value, const SList& L) {
+  __shared__ int s_cnt[16];
+  __shared__ long long s_base;
+  const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const unsigned long long m = ballot(take);
+  if (lane_id() == 0) s_cnt[wv] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < nw; ++w) tot += s_cnt[w];
+    s_base = tot ? (long long)atomicAdd(&L.cnt[blockIdx.x & (kStripes - 1)],
+                                        (unsigned long long)tot)
+                 : 0;
+  }
+  __syncthreads();
+  long long base = s_base;
+  for (int w = 0; w < wv; ++w) base += s_cnt[w];
+  if (take) L.data[(int64_t)(blockIdx.x & (kStripes - 1)) * L.cap + base + mbcnt(m)] = value;
+  __syncthreads();  // s_cnt / s_base are reused by the next call
 }
 
 // Prefix of the sub-list counts of L in LDS (s_pre[kStripes + 1]); returns

@@ -20,6 +20,8 @@ void launch_rows(const Dev& d, int64_t n, int grid_cells, hipStream_t s);
 void launch_scan_top(int64_t* v, int nb, hipStream_t s);
 
 int64_t sort_hist_entries(int64_t n);
+// per-stripe capacity of the row pass's work list for up to n cells
+int64_t row_list_cap(int64_t n);
 int64_t scan_tmp_entries(int64_t n);
 
 void launch_core(const Dev& d, bool f32, hipStream_t s);
