@@ -119,30 +119,41 @@ def kernel_bytes(st, dim, f32):
              per listed core point of a light cell the list entry 4.
     k_adj    per lane (core point of a light cell): list entry 4, cell_of 4,
              cell_z 4, fixed-point coordinates 4*D (+ FP64 8*D in the FP64
-             form), 3^(D-1) packed near rows; per non-empty near line its first
-             start 4; per cell on it: next start, cell_z, core count (12); per
-             pair test: core flag 1 + coordinates s*D; per core cell the mask 8.
-    k_rows   per cell and stencil line: line-table entry 8, packed row
-             written 4; per cell and group of 5 lines: cell_xyz 8 (the
-             last-coordinate loads of its searches are not counted: a lower
-             bound).
+             form), the packed rows of the near lines above the cell (5 in
+             3-D, 2 in 2-D); per non-empty near line its first start 4; per
+             cell on it: next start, cell_z, core count (12); per pair test:
+             core flag 1 + coordinates s*D; per core cell the mask 8; per
+             adjacency bit found (half the directed bits, diag[3] / 2, exact
+             when no cell is heavy) the mirror: the row of each cell 8 and
+             the other cell's mask 8.
+    k_rows   per cell and group of 5 lines: cell_xyz 8, the cell's sort key
+             staged 8, five packed rows written 20 (the line-table entries
+             of a tile's span and the keys of the extra lines around it are
+             not counted: a lower bound).
+    k_expand per popped cell: chain label 4, adjacency mask 8, 9 near rows
+             36; per adjacency bit: the neighbour's label 4 and its claim 4;
+             per cell the seeding scan: core count 4 and label 4.
     s = 4 (fixed point, FP32 form) or 8 (FP64 form)."""
     n, c = st["n_points"], st["n_cells"]
     s = 4 if f32 else 8
     w = st["work"]
     lines_all = 25 if dim == 3 else 5
-    near = 9 if dim == 3 else 3
+    upper = 5 if dim == 3 else 2
     groups = lines_all // 5
     a = st["n_adj_points"]
+    dg = st["diag"]
     core = (n * (4 + 8 + s * dim + 1) + 20 * w[0] + 8 * w[1] + s * dim * st["pairs_evaluated"] +
             c * (4 + 8) + 4 * a)
-    lane = 12 + 4 * dim + (0 if f32 else 8 * dim) + 4 * near
-    adj = a * lane + 4 * w[2] + 12 * w[3] + st["diag"][8] * (1 + s * dim) + 8 * st["n_core_cells"]
-    rows = c * (lines_all * (8 + 4) + groups * 8)
-    return {"k_core": core, "k_adj": adj, "k_rows": rows}
+    lane = 12 + 4 * dim + (0 if f32 else 8 * dim) + 4 * upper
+    adj = (a * lane + 4 * w[2] + 12 * w[3] + dg[8] * (1 + s * dim) + 8 * st["n_core_cells"] +
+           16 * (dg[3] // 2))
+    rows = c * groups * (8 + 8 + 20)
+    expand = dg[0] * (4 + 8 + 36) + dg[3] * 8 + c * 8
+    return {"k_core": core, "k_adj": adj, "k_rows": rows, "k_expand": expand}
 
 
-KERNEL_TIME = {"k_core": "t_core_ms", "k_adj": "t_adj_ms", "k_rows": "t_rows_ms"}
+KERNEL_TIME = {"k_core": "t_core_ms", "k_adj": "t_adj_ms", "k_rows": "t_rows_ms",
+               "k_expand": "t_expand_ms"}
 
 
 def measured_traffic(workload, precision):
