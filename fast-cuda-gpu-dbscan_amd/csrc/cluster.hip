@@ -1146,17 +1146,24 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
     const int nb = __popcll(m0);
     edges += nb;
     const int incl = wave_incl_scan(nb);
-    const int total = __shfl(incl, 63, 64);
+    const int total = rfl_i(__shfl(incl, 63, 64));
     pst.own[lane] = own;
     pst.mask[lane] = m0;
     pst.pre[lane] = incl - nb;
     wave_sync();
     constexpr int T = 8;  // items per lane step
     for (int base = 0; base < total; base += 64 * T) {
+      // sub-steps of 64 items that hold any (wave-uniform: a small round,
+      // the usual case late in a chain, skips the empty ones)
+      const int nsub = (total - base + 63) >> 6 < T ? (total - base + 63) >> 6 : T;
       int32_t tg[T], ow[T];
       bool v[T];
 #pragma unroll
       for (int j = 0; j < T; ++j) {
+        v[j] = false;
+        tg[j] = 0;
+        ow[j] = 0;
+        if (j >= nsub) continue;
         const int t = base + j * 64 + lane;
         v[j] = t < total;
         int src = 0;
@@ -1184,12 +1191,12 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
       // once), so only those cells pay for a CAS
       int32_t lab[T];
 #pragma unroll
-      for (int j = 0; j < T; ++j) lab[j] = v[j] ? d.cell_label[tg[j]] : ow[j];
+      for (int j = 0; j < T; ++j) lab[j] = (j < nsub && v[j]) ? d.cell_label[tg[j]] : ow[j];
       bool cl[T];
 #pragma unroll
       for (int j = 0; j < T; ++j) {
         cl[j] = false;
-        if (v[j] && lab[j] == kUnclaimed) {
+        if (j < nsub && v[j] && lab[j] == kUnclaimed) {
           const int32_t old = atomicCAS(&d.cell_label[tg[j]], kUnclaimed, ow[j]);
           cl[j] = old == kUnclaimed;
           lab[j] = cl[j] ? ow[j] : old;
@@ -1197,6 +1204,7 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
       }
 #pragma unroll
       for (int j = 0; j < T; ++j) {
+        if (j >= nsub) break;
         // pushes: one slot range per step
         const unsigned long long mc = ballot(cl[j]);
         if (mc) {
