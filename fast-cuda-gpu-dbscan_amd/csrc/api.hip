@@ -339,14 +339,15 @@ hipError_t enqueue_core(dclust_handle* h, const Dev& d, const RunShape& shape, b
   if ((e = mark(h, E_CORE, graph)) != hipSuccess) return e;
   launch_adj(d, f32, h->num_cu * 16, s);
   if ((e = mark(h, E_ADJ, graph)) != hipSuccess) return e;
-  launch_seg_init(d, shape.waves, s);
   return hipSuccess;
 }
 
 void enqueue_pass(const Dev& d, const RunShape& shape, int pass, hipStream_t s) {
-  launch_pass_prep(d, pass, s);
+  // (pass 0 needs no preparation: the counters it would reset are zero from
+  // the start of the run; it drains no queue, so nothing is carried)
+  if (pass > 0) launch_pass_prep(d, pass, s);
   launch_expand(d, pass, shape.blocks, shape.seeds, shape.threads, shape.budget, s);
-  launch_pass_carry(d, pass, s);
+  if (pass > 0) launch_pass_carry(d, pass, s);
 }
 
 // chain merge, far pairs, border points, canonical labels

@@ -1309,14 +1309,6 @@ __global__ __launch_bounds__(256) void k_pass_carry(Dev d, int pass) {
   for (long long i = head + threadIdx.x; i < qn; i += blockDim.x) qout[s_base + (i - head)] = qin[i];
 }
 
-// Segment start of every expansion wave (seeding cursor over the cells).
-__global__ void k_seg_init(Dev d, int64_t W) {
-  const int64_t C = d.ctl[C_NCELL];
-  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < W;
-       w += (int64_t)gridDim.x * blockDim.x)
-    d.seg_pos[w] = C * w / W;
-}
-
 // ----------------------------------------------------------------- merge --
 // Chain ids are seed cell indices.  When the id space fits in LDS (up to
 // 32768 cells), one workgroup merges every recorded pair on an LDS copy of
@@ -1632,12 +1624,14 @@ __global__ __launch_bounds__(256) void k_output(Dev d) {
   stat_add(d, C_NNOISE, noise);
 }
 
-// Per-run state of the clustering phases, one launch: chain labels of the
-// cells (unclaimed), the chain forest (every id its own root), component
-// minima, component flags, per-cell core counts (all over the n entries
-// that bound the cells and chain ids) and the per-wave pair counts.
+// Per-run state of the clustering phases, one launch after the index:
+// chain labels of the cells (unclaimed), the chain forest (every id its own
+// root), component minima, component flags, per-cell core counts (all over
+// the n entries that bound the cells and chain ids), and per expansion
+// wave its pair count and the start of its seeding segment of the cells.
 __global__ void k_init_run(Dev d, int64_t waves) {
   const int64_t m = d.n > waves ? d.n : waves;
+  const int64_t C = d.ctl[C_NCELL];
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
        i += (int64_t)gridDim.x * blockDim.x) {
     if (i < d.n) {
@@ -1647,7 +1641,10 @@ __global__ void k_init_run(Dev d, int64_t waves) {
       d.flag[i] = 0;
       d.cell_ncore[i] = 0;
     }
-    if (i < waves) d.pair_cnt[i] = 0;
+    if (i < waves) {
+      d.pair_cnt[i] = 0;
+      d.seg_pos[i] = C * i / waves;
+    }
   }
 }
 
@@ -1696,10 +1693,6 @@ void launch_pass_carry(const Dev& d, int pass, hipStream_t s) {
 
 void launch_expand_check(const Dev& d, int passes, hipStream_t s) {
   hipLaunchKernelGGL(k_expand_check, dim3(1), dim3(1), 0, s, d, passes);
-}
-
-void launch_seg_init(const Dev& d, int64_t waves, hipStream_t s) {
-  hipLaunchKernelGGL(k_seg_init, dim3(64), dim3(256), 0, s, d, waves);
 }
 
 void launch_expand(const Dev& d, int pass, int grid, int seeds, int threads, int budget,

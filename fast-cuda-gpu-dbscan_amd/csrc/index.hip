@@ -497,7 +497,14 @@ __device__ __forceinline__ const uint64_t* sorted_keys(const Dev& d) {
   return d.key[sort_parity(d, kMaxPasses)];
 }
 
-__global__ __launch_bounds__(CS_T) void k_csr_count(Dev d) {
+// (also, after a fixed number of radix passes: a key wider than they cover
+// leaves the keys unsorted, so the run is abandoned: every later kernel
+// returns at once and the host runs the call again; counting unsorted keys
+// here is harmless)
+__global__ __launch_bounds__(CS_T) void k_csr_count(Dev d, int passes) {
+  if (blockIdx.x == 0 && threadIdx.x == 0 && passes < kMaxPasses &&
+      d.ctl[C_KEYBITS] > 8 * passes)
+    d.ctl[C_ABANDON] = 1;
   if (abandoned(d)) return;
   const uint64_t* key = sorted_keys(d);
   const int64_t t0 = (int64_t)blockIdx.x * CS_TILE;
@@ -1021,12 +1028,6 @@ __global__ __launch_bounds__(256) void k_rows_global(Dev d) {
   stat_add(d, C_NBRC, nbrc);
 }
 
-// After a fixed number of radix passes: a key wider than they cover leaves
-// the keys unsorted, so the run is abandoned (every later kernel returns at
-// once and the host runs the call again).
-__global__ void k_sort_check(Dev d, int passes) {
-  if (d.ctl[C_KEYBITS] > 8 * passes) d.ctl[C_ABANDON] = 1;
-}
 
 // ------------------------------------------------------------- launchers --
 void launch_index_head(const Dev& d, int nchunks, hipStream_t s) {
@@ -1044,9 +1045,8 @@ void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
     hipLaunchKernelGGL(k_rs_rowscan, dim3(256), dim3(256), 0, s, d, p, nblk);
     hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
   }
-  if (passes < kMaxPasses) hipLaunchKernelGGL(k_sort_check, dim3(1), dim3(1), 0, s, d, passes);
   const int ncs = (int)((n + CS_TILE - 1) / CS_TILE);
-  hipLaunchKernelGGL(k_csr_count, dim3(ncs), dim3(CS_T), 0, s, d);
+  hipLaunchKernelGGL(k_csr_count, dim3(ncs), dim3(CS_T), 0, s, d, passes);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, d.scan_tmp, ncs, (long long*)nullptr);
   hipLaunchKernelGGL(k_csr_write, dim3(ncs), dim3(CS_T), 0, s, d);
   // random-access gather: more blocks in flight than a fill needs

@@ -28,7 +28,6 @@ void launch_core(const Dev& d, bool f32, hipStream_t s);
 void launch_adj(const Dev& d, bool f32, int heavy_grid, hipStream_t s);
 void launch_pass_prep(const Dev& d, int pass, hipStream_t s);
 void launch_pass_carry(const Dev& d, int pass, hipStream_t s);
-void launch_seg_init(const Dev& d, int64_t waves, hipStream_t s);
 // after `passes` expansion passes: pending work abandons the rest of the run
 void launch_expand_check(const Dev& d, int passes, hipStream_t s);
 void launch_expand(const Dev& d, int pass, int grid, int seeds, int threads, int budget,
