@@ -30,6 +30,8 @@ namespace {
 
 constexpr int64_t kChunk = 4096;  // points per chunk of bbox / cell-key blocks
 constexpr int kMaxExpandPasses = 100000;
+// heavy cells from which their adjacency pass runs beside the light one
+constexpr long long kHeavyOverlap = 1024;
 
 struct Buf {
   void* p = nullptr;
@@ -47,6 +49,10 @@ struct dclust_handle {
   int device = 0;
   int num_cu = 0;
   hipStream_t stream = nullptr;
+  // second stream (default priority, like the first) for kernels that run
+  // beside the main sequence, and the events that fork and join it
+  hipStream_t stream2 = nullptr;
+  hipEvent_t fork[4] = {};
   hipEvent_t ev[E_N] = {};
   std::vector<Buf> bufs;
   int64_t last_n = -1;
@@ -61,6 +67,7 @@ struct dclust_handle {
   std::vector<uint8_t> prev_sig;   // launch state of the last eager run
   int prev_npass = 0;              // its expansion passes
   long long prev_nlines = 0;       // its neighbour-line count
+  long long prev_nheavy = 0;       // heavy cells of the last run (any kind)
   int prev_sort_passes = 0;        // its radix passes
   hipGraphExec_t gexec = nullptr;  // captured run
   std::vector<uint8_t> gsig;       // launch state it was captured for
@@ -315,9 +322,38 @@ std::vector<uint8_t> launch_signature(const Dev& d, const RunShape& r, bool f32,
 }
 
 // Phase event; inside a stream capture it becomes a node of the graph.
-hipError_t mark(dclust_handle* h, int e, bool graph) {
-  return graph ? hipEventRecordWithFlags(h->ev[e], h->stream, hipEventRecordExternal)
-               : hipEventRecord(h->ev[e], h->stream);
+hipError_t mark(dclust_handle* h, int e, bool graph, hipStream_t s = nullptr) {
+  if (!s) s = h->stream;
+  return graph ? hipEventRecordWithFlags(h->ev[e], s, hipEventRecordExternal)
+               : hipEventRecord(h->ev[e], s);
+}
+
+// Work on the second stream after everything enqueued so far on the first
+// (fork event k), and the first stream's wait for it (join event k + 1).
+hipError_t fork_second(dclust_handle* h, int k) {
+  hipError_t e = hipEventRecord(h->fork[k], h->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(h->stream2, h->fork[k], 0);
+  return e;
+}
+hipError_t join_second(dclust_handle* h, int k) {
+  hipError_t e = hipEventRecord(h->fork[k + 1], h->stream2);
+  if (e == hipSuccess) e = hipStreamWaitEvent(h->stream, h->fork[k + 1], 0);
+  return e;
+}
+
+// Keys, sort, CSR, gather and line table, then the neighbour rows (one
+// stream: the gather fills the device, so running the rows beside it gains
+// nothing).
+hipError_t enqueue_index(dclust_handle* h, const Dev& d, int64_t n, int nchunks, int grid_cells,
+                         int passes, bool graph) {
+  const hipStream_t s = h->stream;
+  hipError_t e;
+  launch_index_sort(d, n, nchunks, passes, s);
+  launch_gather(d, n, s);
+  launch_lines(d, grid_cells, s);
+  if ((e = mark(h, E_ROWS0, graph)) != hipSuccess) return e;
+  launch_rows(d, n, grid_cells, s);
+  return hipSuccess;
 }
 
 // counters, bounding boxes, grid set-up
@@ -337,7 +373,17 @@ hipError_t enqueue_core(dclust_handle* h, const Dev& d, const RunShape& shape, b
   if ((e = mark(h, E_CORE0, graph)) != hipSuccess) return e;
   launch_core(d, f32, s);
   if ((e = mark(h, E_CORE, graph)) != hipSuccess) return e;
-  launch_adj(d, f32, h->num_cu * 16, s);
+  // heavy cells on the second stream beside the light ones when the last
+  // run had many (the fork and join cost more than a near-empty heavy pass
+  // takes); the labels do not depend on it
+  if (h->prev_nheavy >= kHeavyOverlap) {
+    if ((e = fork_second(h, 2)) != hipSuccess) return e;
+    launch_adj_heavy(d, f32, h->num_cu * 16, h->stream2);
+    launch_adj_light(d, f32, s);
+    if ((e = join_second(h, 2)) != hipSuccess) return e;
+  } else {
+    launch_adj(d, f32, h->num_cu * 16, s);
+  }
   if ((e = mark(h, E_ADJ, graph)) != hipSuccess) return e;
   return hipSuccess;
 }
@@ -380,13 +426,9 @@ hipError_t capture_run(dclust_handle* h, const Dev& d, const RunShape& shape, bo
   hipError_t body = mark(h, E_START, true);
   if (body == hipSuccess) {
     enqueue_head(d, nchunks, s);
-    launch_index_tail(d, d.n, nchunks, grid_cells, sort_passes, s);
-    body = mark(h, E_ROWS0, true);
+    body = enqueue_index(h, d, d.n, nchunks, grid_cells, sort_passes, true);
   }
-  if (body == hipSuccess) {
-    launch_rows(d, d.n, grid_cells, s);
-    body = enqueue_core(h, d, shape, f32, true);
-  }
+  if (body == hipSuccess) body = enqueue_core(h, d, shape, f32, true);
   for (int p = 0; body == hipSuccess && p < passes; ++p) {
     enqueue_pass(d, shape, p, s);
     if (p == 0) body = mark(h, E_PASS0, true);
@@ -443,6 +485,7 @@ void finish_run(dclust_handle* h, const Dev& d, const RunShape& shape, long long
   for (int c = 0; c < C_NCTL; ++c)
     for (int k = 0; k < kStripes; ++k) ctl_h[c] += stat_h[(size_t)c * kStripes + k];
   h->last_n = d.n;
+  h->prev_nheavy = ctl_h[C_NHEAVY];
   h->last_core = d.core;
   h->last_orig = d.orig;
   h->last_core_out = d.core_out;
@@ -772,9 +815,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
       // (a fixed region without room for the table: the rows are found by
       // binary search over the cell keys instead)
     }
-    launch_index_tail(d, ntot, nchunks, grid_cells, passes, s);
-    HIPCHK(mark(h, E_ROWS0, false));
-    launch_rows(d, ntot, grid_cells, s);
+    HIPCHK(enqueue_index(h, d, ntot, nchunks, grid_cells, passes, false));
     if (onestep) {
       HIPCHK(mark(h, E_INDEX, false));
       return onestep_loop(h, d, params[0], onestep_threads, d_labels, stats);
@@ -1039,13 +1080,20 @@ int dclust_create(dclust_handle** out, int device) {
     (void)hipGetLastError();
     h->scratch = nullptr;
   }
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess) {
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return DCLUST_E_HIP;
   }
   for (auto& e : h->ev)
     if (hipEventCreate(&e) != hipSuccess) {
-      delete h;
+      dclust_destroy(h);
+      return DCLUST_E_HIP;
+    }
+  for (auto& e : h->fork)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      dclust_destroy(h);
       return DCLUST_E_HIP;
     }
   *out = h;
@@ -1056,11 +1104,15 @@ void dclust_destroy(dclust_handle* h) {
   if (!h) return;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->stream2) (void)hipStreamSynchronize(h->stream2);
   drop_graph(h);
   for (auto& b : h->bufs)
     if (b.p) (void)hipFree(b.p);
   for (auto& e : h->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : h->fork)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream2) (void)hipStreamDestroy(h->stream2);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }

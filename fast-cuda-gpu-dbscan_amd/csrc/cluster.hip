@@ -1673,14 +1673,23 @@ void launch_core(const Dev& d, bool f32, hipStream_t s) {
     hipLaunchKernelGGL(k_core<false>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
 }
 
-void launch_adj(const Dev& d, bool f32, int heavy_grid, hipStream_t s) {
-  if (f32) {
+void launch_adj_light(const Dev& d, bool f32, hipStream_t s) {
+  if (f32)
     hipLaunchKernelGGL(k_adj<true>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_adj_heavy<true>, dim3(heavy_grid), dim3(64), 0, s, d);
-  } else {
+  else
     hipLaunchKernelGGL(k_adj<false>, dim3(point_grid(d.n)), dim3(256), 0, s, d);
+}
+
+void launch_adj_heavy(const Dev& d, bool f32, int heavy_grid, hipStream_t s) {
+  if (f32)
+    hipLaunchKernelGGL(k_adj_heavy<true>, dim3(heavy_grid), dim3(64), 0, s, d);
+  else
     hipLaunchKernelGGL(k_adj_heavy<false>, dim3(heavy_grid), dim3(64), 0, s, d);
-  }
+}
+
+void launch_adj(const Dev& d, bool f32, int heavy_grid, hipStream_t s) {
+  launch_adj_light(d, f32, s);
+  launch_adj_heavy(d, f32, heavy_grid, s);
 }
 
 void launch_pass_prep(const Dev& d, int pass, hipStream_t s) {

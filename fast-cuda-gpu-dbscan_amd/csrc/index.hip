@@ -1035,8 +1035,7 @@ void launch_index_head(const Dev& d, int nchunks, hipStream_t s) {
   hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(1024), 0, s, d);
 }
 
-void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
-                       int passes, hipStream_t s) {
+void launch_index_sort(const Dev& d, int64_t n, int nchunks, int passes, hipStream_t s) {
   hipLaunchKernelGGL(k_cellkey, dim3(nchunks), dim3(256), 0, s, d);
   const int nblk = (int)((n + RS_TILE - 1) / RS_TILE);
   for (int p = 0; p < passes; ++p) {
@@ -1049,12 +1048,25 @@ void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
   hipLaunchKernelGGL(k_csr_count, dim3(ncs), dim3(CS_T), 0, s, d, passes);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, d.scan_tmp, ncs, (long long*)nullptr);
   hipLaunchKernelGGL(k_csr_write, dim3(ncs), dim3(CS_T), 0, s, d);
+}
+
+void launch_gather(const Dev& d, int64_t n, hipStream_t s) {
   // random-access gather: more blocks in flight than a fill needs
   const int64_t gb = (n + 255) / 256;
   hipLaunchKernelGGL(k_gather, dim3((int)std::max<int64_t>(1, std::min<int64_t>(16384, gb))),
                      dim3(256), 0, s, d);
+}
+
+void launch_lines(const Dev& d, int grid_cells, hipStream_t s) {
   hipLaunchKernelGGL(k_line_clear, dim3(grid_cells), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_line_mark, dim3(grid_cells), dim3(256), 0, s, d);
+}
+
+void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
+                       int passes, hipStream_t s) {
+  launch_index_sort(d, n, nchunks, passes, s);
+  launch_gather(d, n, s);
+  launch_lines(d, grid_cells, s);
 }
 
 // Row pass grid: one wavefront per (tile of 64 cells, line group) for up

@@ -14,6 +14,11 @@ void launch_index_head(const Dev& d, int nchunks, hipStream_t s);
 // cell keys, `passes` radix passes, CSR compaction, gather, line table
 void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
                        int passes, hipStream_t s);
+// the same in three parts: keys, sort and CSR; the gather of the
+// coordinates; the line table (the last two are independent)
+void launch_index_sort(const Dev& d, int64_t n, int nchunks, int passes, hipStream_t s);
+void launch_gather(const Dev& d, int64_t n, hipStream_t s);
+void launch_lines(const Dev& d, int grid_cells, hipStream_t s);
 // neighbour rows of every cell (k_rows)
 void launch_rows(const Dev& d, int64_t n, int grid_cells, hipStream_t s);
 // exclusive in-place scan of nb int64 values by one block
@@ -26,6 +31,10 @@ int64_t scan_tmp_entries(int64_t n);
 
 void launch_core(const Dev& d, bool f32, hipStream_t s);
 void launch_adj(const Dev& d, bool f32, int heavy_grid, hipStream_t s);
+// its two kernels apart: light cells (k_adj) and heavy cells (k_adj_heavy)
+// are independent (both only OR bits into the masks)
+void launch_adj_light(const Dev& d, bool f32, hipStream_t s);
+void launch_adj_heavy(const Dev& d, bool f32, int heavy_grid, hipStream_t s);
 void launch_pass_prep(const Dev& d, int pass, hipStream_t s);
 void launch_pass_carry(const Dev& d, int pass, hipStream_t s);
 // after `passes` expansion passes: pending work abandons the rest of the run
