@@ -38,7 +38,7 @@ static int fill_grid(int64_t n) {
 __global__ void k_init_head(Dev d) {
   const int64_t nb = (int64_t)d.ninst * kBoxStripes * 8;
   const int64_t ns = (int64_t)C_NCTL * kStripes;
-  const int64_t tot = nb + ns + C_NCTL + kMaxPasses;
+  const int64_t tot = nb + ns + C_NCTL + kMaxPasses + d.ninst;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
        i += (int64_t)gridDim.x * blockDim.x) {
     if (i < nb)
@@ -47,14 +47,16 @@ __global__ void k_init_head(Dev d) {
       d.stat[i - nb] = 0;
     else if (i < nb + ns + C_NCTL)
       d.ctl[i - nb - ns] = 0;
-    else
+    else if (i < nb + ns + C_NCTL + kMaxPasses)
       d.pass_scatter[i - nb - ns - C_NCTL] = 0;
+    else
+      d.inst[i - nb - ns - C_NCTL - kMaxPasses].err = 0;  // (the table may be last run's)
   }
 }
 
 void launch_init_head(const Dev& d, hipStream_t s) {
   const int64_t tot = (int64_t)d.ninst * kBoxStripes * 8 + (int64_t)C_NCTL * kStripes + C_NCTL +
-                      kMaxPasses;
+                      kMaxPasses + d.ninst;
   hipLaunchKernelGGL(k_init_head, dim3(fill_grid(tot)), dim3(256), 0, s, d);
 }
 
