@@ -1565,14 +1565,22 @@ constexpr int RK_T = 256;
 constexpr int RK_I = 16;
 constexpr int RK_TILE = RK_T * RK_I;
 
+// component-minimum flag of input index i (false past the end); the load is
+// unconditional so that a thread's loads for all of its rounds overlap
+__device__ __forceinline__ bool rank_flag(const Dev& d, int64_t i) {
+  const uint8_t f = d.flag[i < d.n ? i : 0];
+  return i < d.n && f;
+}
+
 __global__ __launch_bounds__(RK_T) void k_rank_count(Dev d) {
   if (abandoned(d)) return;
   const int64_t t0 = (int64_t)blockIdx.x * RK_TILE;
+  bool f[RK_I];
+#pragma unroll
+  for (int r = 0; r < RK_I; ++r) f[r] = rank_flag(d, t0 + r * RK_T + threadIdx.x);
   int c = 0;
-  for (int r = 0; r < RK_I; ++r) {
-    int64_t i = t0 + r * RK_T + threadIdx.x;
-    c += __popcll(ballot(i < d.n && d.flag[i]));
-  }
+#pragma unroll
+  for (int r = 0; r < RK_I; ++r) c += __popcll(ballot(f[r]));
   __shared__ int s[4];
   if (lane_id() == 0) s[threadIdx.x >> 6] = c;
   __syncthreads();
@@ -1585,11 +1593,15 @@ __global__ __launch_bounds__(RK_T) void k_rank_write(Dev d) {
   __shared__ int s_w[4];
   __shared__ int64_t s_run;
   const int wv = threadIdx.x >> 6;
+  bool fr[RK_I];
+#pragma unroll
+  for (int r = 0; r < RK_I; ++r) fr[r] = rank_flag(d, t0 + r * RK_T + threadIdx.x);
   if (threadIdx.x == 0) s_run = d.scan_tmp[blockIdx.x];
   __syncthreads();
+#pragma unroll
   for (int r = 0; r < RK_I; ++r) {
     int64_t i = t0 + r * RK_T + threadIdx.x;
-    bool f = i < d.n && d.flag[i];
+    bool f = fr[r];
     unsigned long long m = ballot(f);
     if (lane_id() == 0) s_w[wv] = __popcll(m);
     __syncthreads();

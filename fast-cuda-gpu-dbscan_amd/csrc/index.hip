@@ -347,17 +347,24 @@ __device__ __forceinline__ bool pass_active(const Dev& d, int pass) {
 __global__ __launch_bounds__(RS_T) void k_rs_hist(Dev d, int pass, int nblk) {
   if (!pass_active(d, pass)) return;
   const uint64_t* keys = d.key[sort_parity(d, pass)];
+  const int shift = pass * 8;
+  const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
+  // all of the thread's keys are loaded at once (an index past the end reads
+  // key 0 and is not counted), so their latencies overlap
+  uint32_t dg[RS_I];
+#pragma unroll
+  for (int r = 0; r < RS_I; ++r) {
+    const int64_t i = t0 + r * RS_T + threadIdx.x;
+    dg[r] = (uint32_t)(keys[i < d.n ? i : 0] >> shift) & 255u;
+  }
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
-  const int shift = pass * 8;
-  const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
   for (int r = 0; r < RS_I; ++r) {
-    int64_t i = t0 + r * RS_T + threadIdx.x;
-    bool v = i < d.n;
-    uint32_t dg = v ? (uint32_t)(keys[i] >> shift) & 255u : 0u;
-    unsigned long long peers = match8(dg, v);
-    if (v && mbcnt(peers) == 0) atomicAdd(&h[dg], (uint32_t)__popcll(peers));
+    const bool v = t0 + r * RS_T + threadIdx.x < d.n;
+    const unsigned long long peers = match8(dg[r], v);
+    if (v && mbcnt(peers) == 0) atomicAdd(&h[dg[r]], (uint32_t)__popcll(peers));
   }
   __syncthreads();
   d.hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
@@ -436,6 +443,18 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) 
   __shared__ uint32_t s_wc[4][256];
   const int tid = threadIdx.x, wv = tid >> 6;
   const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
+  // the tile's keys and values are loaded at once, before the digit
+  // offsets, so that their latencies overlap (an index past the end reads
+  // element 0, which is not ranked)
+  uint64_t kr[RS_I];
+  int32_t vr[RS_I];
+#pragma unroll
+  for (int r = 0; r < RS_I; ++r) {
+    const int64_t i = t0 + r * RS_T + tid;
+    const int64_t ic = i < d.n ? i : 0;
+    kr[r] = kin[ic];
+    vr[r] = vin[ic];
+  }
   {
     uint32_t c = d.hist[(int64_t)tid * nblk + blockIdx.x];
     s_goff[tid] = d.offs[(int64_t)tid * nblk + blockIdx.x];
@@ -452,11 +471,11 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) 
     s_lstart[tid] -= c;
     __syncthreads();
   }
+#pragma unroll
   for (int r = 0; r < RS_I; ++r) {
-    int64_t i = t0 + r * RS_T + tid;
-    bool v = i < d.n;
-    uint64_t key = v ? kin[i] : 0;
-    int32_t val = v ? vin[i] : 0;
+    const bool v = t0 + r * RS_T + tid < d.n;
+    const uint64_t key = kr[r];
+    const int32_t val = vr[r];
     uint32_t dg = (uint32_t)(key >> shift) & 255u;
     unsigned long long peers = match8(dg, v);
     int rank = mbcnt(peers);
@@ -499,6 +518,21 @@ __device__ __forceinline__ const uint64_t* sorted_keys(const Dev& d) {
   return d.key[sort_parity(d, kMaxPasses)];
 }
 
+// Sorted keys i and i - 1 of each of a thread's rounds, loaded before any
+// is used (indices clamped into the array) so that the loads are in flight
+// together; key i starts a cell when i < n and it differs from key i - 1.
+template <int NI>
+__device__ __forceinline__ void load_key_pairs(const Dev& d, const uint64_t* key, int64_t t0,
+                                               uint64_t (&k)[NI], uint64_t (&kp)[NI]) {
+#pragma unroll
+  for (int r = 0; r < NI; ++r) {
+    const int64_t i = t0 + r * CS_T + threadIdx.x;
+    const int64_t ic = i < d.n ? i : 0;
+    k[r] = key[ic];
+    kp[r] = key[ic > 0 ? ic - 1 : 0];
+  }
+}
+
 // (also, after a fixed number of radix passes: a key wider than they cover
 // leaves the keys unsorted, so the run is abandoned: every later kernel
 // returns at once and the host runs the call again; counting unsorted keys
@@ -510,12 +544,13 @@ __global__ __launch_bounds__(CS_T) void k_csr_count(Dev d, int passes) {
   if (abandoned(d)) return;
   const uint64_t* key = sorted_keys(d);
   const int64_t t0 = (int64_t)blockIdx.x * CS_TILE;
+  uint64_t kr[CS_I], kp[CS_I];
+  load_key_pairs<CS_I>(d, key, t0, kr, kp);
   int c = 0;
+#pragma unroll
   for (int r = 0; r < CS_I; ++r) {
-    int64_t i = t0 + r * CS_T + threadIdx.x;
-    bool f = i < d.n && (i == 0 || key[i] != key[i - 1]);
-    unsigned long long m = ballot(f);
-    c += __popcll(m);
+    const int64_t i = t0 + r * CS_T + threadIdx.x;
+    c += __popcll(ballot(i < d.n && (i == 0 || kr[r] != kp[r])));
   }
   __shared__ int s[4];
   if (lane_id() == 0) s[threadIdx.x >> 6] = c;
@@ -591,13 +626,16 @@ __global__ __launch_bounds__(CS_T) void k_csr_write(Dev d) {
   __shared__ int s_w[4];
   __shared__ int64_t s_run;
   const int wv = threadIdx.x >> 6;
+  uint64_t kr[CS_I], kp[CS_I];
+  load_key_pairs<CS_I>(d, key, t0, kr, kp);
   if (threadIdx.x == 0) s_run = d.scan_tmp[blockIdx.x];
   __syncthreads();
+#pragma unroll
   for (int r = 0; r < CS_I; ++r) {
     int64_t i = t0 + r * CS_T + threadIdx.x;
     bool inr = i < d.n;
-    uint64_t k = inr ? key[i] : 0;
-    bool f = inr && (i == 0 || k != key[i - 1]);
+    const uint64_t k = kr[r];
+    const bool f = inr && (i == 0 || k != kp[r]);
     unsigned long long m = ballot(f);
     if (lane_id() == 0) s_w[wv] = __popcll(m);
     __syncthreads();
