@@ -65,6 +65,34 @@ void launch_init_head(const Dev& d, hipStream_t s) {
 // One block per chunk (chunks never straddle instances).  Block reduction,
 // then one ordered-u64 atomic min/max per axis.  Non-finite input sets the
 // instance error.
+// Points of a chunk per thread step: the loads of kChunkU points (strided
+// by the block) are issued before any is used, so their latencies overlap.
+constexpr int kChunkU = 4;
+
+template <int D>
+__device__ __forceinline__ void bbox_points(const double* p0, int64_t np, double (&mn)[3],
+                                            double (&mx)[3], double& ma, bool& bad) {
+  for (int64_t i0 = threadIdx.x; i0 < np; i0 += kChunkU * 256) {
+    double v[kChunkU][D];
+#pragma unroll
+    for (int u = 0; u < kChunkU; ++u) {
+      // past the end: the chunk's last point again (min and max unchanged)
+      const int64_t i = i0 + u * 256 < np ? i0 + u * 256 : np - 1;
+#pragma unroll
+      for (int k = 0; k < D; ++k) v[u][k] = p0[i * D + k];
+    }
+#pragma unroll
+    for (int u = 0; u < kChunkU; ++u)
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        bad |= !isfinite(v[u][k]);
+        mn[k] = fmin(mn[k], v[u][k]);
+        mx[k] = fmax(mx[k], v[u][k]);
+        ma = fmax(ma, fabs(v[u][k]));
+      }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_bbox(Dev d) {
   const Chunk ch = d.chunks[blockIdx.x];
   const Inst& I = d.inst[ch.inst];
@@ -73,16 +101,11 @@ __global__ __launch_bounds__(256) void k_bbox(Dev d) {
   double mx[3] = {-INFINITY, -INFINITY, -INFINITY};
   double ma = 0.0;
   bool bad = false;
-  for (int64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
-    const double* p = d.xyz + I.xyz_off + (i - I.pt_begin) * dim;
-    for (int k = 0; k < dim; ++k) {
-      double v = p[k];
-      bad |= !isfinite(v);
-      mn[k] = fmin(mn[k], v);
-      mx[k] = fmax(mx[k], v);
-      ma = fmax(ma, fabs(v));
-    }
-  }
+  const double* p0 = d.xyz + I.xyz_off + (ch.begin - I.pt_begin) * dim;
+  if (dim == 3)
+    bbox_points<3>(p0, ch.end - ch.begin, mn, mx, ma, bad);
+  else
+    bbox_points<2>(p0, ch.end - ch.begin, mn, mx, ma, bad);
   __shared__ double s_red[4][7];
   __shared__ int s_bad;
   if (threadIdx.x == 0) s_bad = 0;
@@ -300,17 +323,41 @@ __device__ __forceinline__ int32_t cell_index(double x, const Inst& I, int k) {
   return c;
 }
 
+template <int D>
+__device__ __forceinline__ void cellkey_points(const Dev& d, const Inst& I, const Chunk& ch) {
+  const double* p0 = d.xyz + I.xyz_off + (ch.begin - I.pt_begin) * D;
+  const int64_t np = ch.end - ch.begin;
+  for (int64_t i0 = threadIdx.x; i0 < np; i0 += kChunkU * 256) {
+    double v[kChunkU][D];
+#pragma unroll
+    for (int u = 0; u < kChunkU; ++u) {
+      const int64_t i = i0 + u * 256 < np ? i0 + u * 256 : np - 1;
+#pragma unroll
+      for (int k = 0; k < D; ++k) v[u][k] = p0[i * D + k];
+    }
+#pragma unroll
+    for (int u = 0; u < kChunkU; ++u) {
+      const int64_t i = i0 + u * 256;
+      if (i < np) {
+        int64_t lin = cell_index(v[u][0], I, 0);
+#pragma unroll
+        for (int k = 1; k < D; ++k) lin = lin * I.nk[k] + cell_index(v[u][k], I, k);
+        d.key[0][ch.begin + i] = (uint64_t)(I.key_base + lin);
+        d.val[0][ch.begin + i] = (int32_t)(ch.begin + i);
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
   const Chunk ch = d.chunks[blockIdx.x];
-  const Inst& I = d.inst[ch.inst];
-  const int dim = I.dim;
-  for (int64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
-    const double* p = d.xyz + I.xyz_off + (i - I.pt_begin) * dim;
-    int64_t lin = cell_index(p[0], I, 0);
-    for (int k = 1; k < dim; ++k) lin = lin * I.nk[k] + cell_index(p[k], I, k);
-    d.key[0][i] = (uint64_t)(I.key_base + lin);
-    d.val[0][i] = (int32_t)i;
-  }
+  // a copy: the key stores could alias the instance table as far as the
+  // compiler knows, which would reload its fields after every store
+  const Inst I = d.inst[ch.inst];
+  if (I.dim == 3)
+    cellkey_points<3>(d, I, ch);
+  else
+    cellkey_points<2>(d, I, ch);
 }
 
 // ------------------------------------------------------------ radix sort --
@@ -665,6 +712,31 @@ __global__ __launch_bounds__(CS_T) void k_csr_write(Dev d) {
 }
 
 // ---------------------------------------------------------------- gather --
+// One point: its whole row is loaded before anything is stored (a store
+// could alias the row as far as the compiler knows, which would otherwise
+// hold each coordinate's random-access load until the previous store).
+template <int D>
+__device__ __forceinline__ void gather_point(const Dev& d, const Inst& I, int64_t i, int32_t src) {
+  const int64_t off = I.xyz_off + ((int64_t)src - I.pt_begin) * D;
+  const double* p = d.xyz + off;
+  double x[D], mn[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    x[k] = p[k];
+    mn[k] = I.mn[k];
+  }
+  const double inv_w = I.inv_w;
+  if (!d.X[0]) d.xoff[i] = off;  // the FP32 form's way back to FP64 rows
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    if (d.X[0]) d.X[k][i] = x[k];  // the FP64 form's sorted copy
+    const double t = cell_t(x[k], mn[k], inv_w);  // same rounding as the key
+    const double f = floor(t * 0x1p29);           // exact scaling by 2^29
+    d.G[k][i] = (uint32_t)(unsigned long long)(long long)f;  // wraps mod 8 cells
+  }
+  d.orig[i] = src;
+}
+
 __global__ __launch_bounds__(256) void k_gather(Dev d) {
   if (abandoned(d)) return;
   const int32_t* vsrc = d.val[sort_parity(d, kMaxPasses)];
@@ -672,18 +744,10 @@ __global__ __launch_bounds__(256) void k_gather(Dev d) {
        i += (int64_t)gridDim.x * blockDim.x) {
     const int32_t src = vsrc[i];
     const Inst& I = *inst_of_cell(d, d.ninst == 1 ? 0 : d.cell_of[i]);
-    const int dim = I.dim;
-    const int64_t off = I.xyz_off + ((int64_t)src - I.pt_begin) * dim;
-    const double* p = d.xyz + off;
-    if (!d.X[0]) d.xoff[i] = off;  // the FP32 form's way back to FP64 rows
-    for (int k = 0; k < dim; ++k) {
-      double x = p[k];
-      if (d.X[0]) d.X[k][i] = x;  // the FP64 form's sorted copy
-      double t = cell_t(x, I.mn[k], I.inv_w);       // same rounding as the key
-      double f = floor(t * 0x1p29);                  // exact scaling by 2^29
-      d.G[k][i] = (uint32_t)(unsigned long long)(long long)f;  // wraps mod 8 cells
-    }
-    d.orig[i] = src;
+    if (I.dim == 3)
+      gather_point<3>(d, I, i, src);
+    else
+      gather_point<2>(d, I, i, src);
   }
 }
 
@@ -713,9 +777,13 @@ __global__ __launch_bounds__(256) void k_line_mark(Dev d) {
   if (d.ctl[C_NLINES] > d.line_cap) return;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < C;
        c += (int64_t)gridDim.x * blockDim.x) {
+    // the three lines are read before either store (which could alias the
+    // cell arrays as far as the compiler knows)
     const int64_t L = cell_line(d, (int32_t)c);
-    if (c == 0 || cell_line(d, (int32_t)c - 1) != L) d.lines[L].x = (int32_t)c;
-    if (c == C - 1 || cell_line(d, (int32_t)c + 1) != L) d.lines[L].y = (int32_t)c + 1;
+    const int64_t Lp = c > 0 ? cell_line(d, (int32_t)c - 1) : -1;
+    const int64_t Ln = c < C - 1 ? cell_line(d, (int32_t)c + 1) : -1;
+    if (Lp != L) d.lines[L].x = (int32_t)c;
+    if (Ln != L) d.lines[L].y = (int32_t)c + 1;
   }
 }
 
