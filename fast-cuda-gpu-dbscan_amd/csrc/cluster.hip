@@ -1441,17 +1441,40 @@ __device__ __forceinline__ int32_t core_chain(const Dev& d, int64_t i) {
   return d.core[i] ? d.cell_label[d.cell_of[i]] : -1;
 }
 
+// Points per thread step of the per-point relabel passes: each level of
+// their dependent loads is issued for all of them before the next level,
+// and nothing is stored until every load is done, so the latencies overlap.
+constexpr int kRelU = 4;
+
 __global__ __launch_bounds__(256) void k_keys(Dev d) {
   if (abandoned(d)) return;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < d.n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t li = core_chain(d, i);
-    if (li >= 0) {
-      const int32_t k = d.compmin[d.parent[li]];
-      d.keyv[i] = k;
-      if (d.orig[i] == k) d.flag[k] = 1;
-    } else {
-      d.keyv[i] = kNoKey;
+  for (int64_t i0 = blockIdx.x * (int64_t)(kRelU * 256); i0 < d.n;
+       i0 += (int64_t)gridDim.x * (kRelU * 256)) {
+    int64_t ic[kRelU];
+    bool cr[kRelU];
+    int32_t c[kRelU], o[kRelU], li[kRelU], pr[kRelU], k[kRelU];
+#pragma unroll
+    for (int u = 0; u < kRelU; ++u) {
+      const int64_t i = i0 + u * 256 + threadIdx.x;
+      ic[u] = i < d.n ? i : 0;
+      cr[u] = i < d.n && d.core[ic[u]];
+      c[u] = d.cell_of[ic[u]];
+      o[u] = d.orig[ic[u]];
+    }
+    // (a core point's cell is claimed by a chain; other lanes read index 0)
+#pragma unroll
+    for (int u = 0; u < kRelU; ++u) li[u] = cr[u] ? d.cell_label[c[u]] : 0;
+#pragma unroll
+    for (int u = 0; u < kRelU; ++u) pr[u] = d.parent[li[u]];
+#pragma unroll
+    for (int u = 0; u < kRelU; ++u) k[u] = d.compmin[pr[u]];
+#pragma unroll
+    for (int u = 0; u < kRelU; ++u) {
+      const int64_t i = i0 + u * 256 + threadIdx.x;
+      if (i < d.n) {
+        d.keyv[i] = cr[u] ? k[u] : kNoKey;
+        if (cr[u] && o[u] == k[u]) d.flag[k[u]] = 1;
+      }
     }
   }
 }
@@ -1621,16 +1644,31 @@ __global__ __launch_bounds__(RK_T) void k_rank_write(Dev d) {
 __global__ __launch_bounds__(256) void k_output(Dev d) {
   if (abandoned(d)) return;
   long long noise = 0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); i0 < d.n; i0 += stride) {
-    const int64_t i = i0 + lane_id();
-    if (i < d.n) {
-      const int32_t o = d.orig[i];
-      const int32_t k = d.keyv[i];
-      const Inst& I = *inst_of_cell(d, d.ninst == 1 ? 0 : d.cell_of[i]);
-      const int32_t lab = k == kNoKey ? 0 : d.rank[k] - d.rank[I.pt_begin] + 1;
-      d.labels_out[o] = lab;
-      noise += lab == 0;
+  for (int64_t i0 = blockIdx.x * (int64_t)(kRelU * 256); i0 < d.n;
+       i0 += (int64_t)gridDim.x * (kRelU * 256)) {
+    int32_t o[kRelU], k[kRelU], rk[kRelU], rb[kRelU];
+    int64_t pb[kRelU];
+#pragma unroll
+    for (int u = 0; u < kRelU; ++u) {
+      const int64_t i = i0 + u * 256 + threadIdx.x;
+      const int64_t ic = i < d.n ? i : 0;
+      o[u] = d.orig[ic];
+      k[u] = d.keyv[ic];
+      pb[u] = inst_of_cell(d, d.ninst == 1 ? 0 : d.cell_of[ic])->pt_begin;
+    }
+#pragma unroll
+    for (int u = 0; u < kRelU; ++u) {
+      rk[u] = d.rank[k[u] == kNoKey ? 0 : k[u]];
+      rb[u] = d.rank[pb[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kRelU; ++u) {
+      const int64_t i = i0 + u * 256 + threadIdx.x;
+      if (i < d.n) {
+        const int32_t lab = k[u] == kNoKey ? 0 : rk[u] - rb[u] + 1;
+        d.labels_out[o[u]] = lab;
+        noise += lab == 0;
+      }
     }
   }
   stat_add(d, C_NNOISE, noise);
