@@ -814,20 +814,24 @@ __global__ __launch_bounds__(1024) void k_cell_comp(Dev d) {
   {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = c < C;
-    const bool hc = in && d.cell_ncore[c] > 0;
-    int32_t chain = -1, cmin = kNoKey;
+    const int64_t cc = in ? c : 0;
+    // every load before the store (which could alias them as far as the
+    // compiler knows), each level of dependent loads issued together
+    const int32_t ncore = d.cell_ncore[cc], q0 = d.cell_start[cc], q1 = d.cell_start[cc + 1];
+    const int32_t lab = d.cell_label[cc];
+    const bool hc = in && ncore > 0;
+    int32_t chain = -1, cmin = kNoKey, comp = -1;
     if (hc) {
-      chain = d.cell_label[c];
-      d.cell_comp[c] = d.parent[chain];
+      chain = lab;
+      comp = d.parent[chain];
       // the sort is stable, so input indices increase within a cell: the
       // cell's smallest core index is that of its first core point
-      int32_t q = d.cell_start[c];
-      if (d.cell_ncore[c] != d.cell_start[c + 1] - q)
+      int32_t q = q0;
+      if (ncore != q1 - q0)
         while (!d.core[q]) ++q;
       cmin = d.orig[q];
-    } else if (in) {
-      d.cell_comp[c] = -1;
     }
+    if (in) d.cell_comp[c] = comp;
     // chain minimum: one atomic per distinct chain of the wave
     bool pend = hc;
     unsigned long long m = ballot(pend);
@@ -987,9 +991,9 @@ __global__ __launch_bounds__(256) void k_roots_final(Dev d) {
   const long long nc = chain_ids(d);
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc;
        c += (long long)gridDim.x * blockDim.x) {
+    const int32_t m = d.compmin[c];
     const int32_t r = uf_root(d.parent, (int32_t)c);
     d.parent[c] = r;
-    const int32_t m = d.compmin[c];
     if (m != kNoKey && r != c) atomicMin(&d.compmin[r], m);
   }
 }
@@ -1525,11 +1529,21 @@ __device__ int32_t border_lane(const Dev& d, const Par& P, int32_t p, int32_t c,
   const int n = R > 0 ? (int)d.rec_n[p] : 256;
   if (n <= R) {
     const int32_t* rec = d.rec + (int64_t)p * R;
-    for (int j = 0; j < n; ++j) {
-      const int32_t q = rec[j];
-      if (!d.core[q]) continue;
-      const int32_t k = d.keyv[q];
-      if (k < best && pred_to<D, F32>(d, P, q, p, pg, px, t)) best = k;
+    // four records at a time: their indices, then their core flags and
+    // keys, each level of loads issued together
+    for (int j0 = 0; j0 < n; j0 += 4) {
+      int32_t q[4], k[4];
+      bool cq[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) q[u] = j0 + u < n ? rec[j0 + u] : p;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cq[u] = j0 + u < n && d.core[q[u]];
+        k[u] = d.keyv[q[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (cq[u] && k[u] < best && pred_to<D, F32>(d, P, q[u], p, pg, px, t)) best = k[u];
     }
     return best;
   }
