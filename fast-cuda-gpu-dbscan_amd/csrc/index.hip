@@ -872,12 +872,15 @@ __device__ __forceinline__ void write_rows(const Dev& d, int32_t c, bool act, in
       }
   }
   if (act) {
+    // the five row indices are read before the first store
+    int r[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
       // (2-D: groups g > 0 write the unused rows 5g .. 5g+4, empty)
-      const int r = dim == 3 ? kRowOfOff3[g][j] : 5 * g + j;
-      d.rows[(int64_t)r * d.rstride + c] = packed[j];
+      r[j] = dim == 3 ? kRowOfOff3[g][j] : 5 * g + j;
     }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) d.rows[(int64_t)r[j] * d.rstride + c] = packed[j];
   }
 }
 
@@ -946,11 +949,18 @@ __device__ __forceinline__ bool rows_staged(const Dev& d, int64_t tile, int g, i
   // (the lines are in cell order: the first and last non-empty entries)
   const int span = lmax >= lmin ? (int)(lmax - lmin + 1) : 0;
   int32_t cmin = -1, cmax = -1;
+  // the entries' loads are all issued before the first is used
+  int2 ev[RW_LINES / 64];
+#pragma unroll
+  for (int k = 0; k < RW_LINES / 64; ++k) {
+    const int i = lane + 64 * k;
+    ev[k] = make_int2(-1, -1);
+    if (64 * k < span && i < span && gl + i < NL) ev[k] = d.lines[gl + i];
+  }
 #pragma unroll
   for (int k = 0; k < RW_LINES / 64; ++k) {
     if (64 * k < span) {
-      const int i = lane + 64 * k;
-      const int2 e = i < span && gl + i < NL ? d.lines[gl + i] : make_int2(-1, -1);
+      const int2 e = ev[k];
       const unsigned long long m = ballot(e.x >= 0 && e.y > e.x);
       if (m) {
         if (cmin < 0) cmin = rl_i(e.x, __ffsll((long long)m) - 1);
@@ -964,9 +974,22 @@ __device__ __forceinline__ bool rows_staged(const Dev& d, int64_t tile, int g, i
   // a power of two > ncz: the last slot is always a ~0 sentinel, so the
   // search (which reaches index pad - 2 at most) is exact
   const int pad = 1 << (32 - __clz(ncz));
-  const uint64_t kbase = (uint64_t)(I.key_base + lmin * nk2);
-  for (int i = lane; i < pad; i += 64)
-    ck[i] = i < ncz ? (uint32_t)(d.cell_key[cmin + i] - kbase) : 0xffffffffu;
+  // keys relative to kbase fit 32 bits, so the low halves of the keys are
+  // enough (little-endian), all loaded before the first is stored
+  const uint32_t kbase = (uint32_t)(I.key_base + lmin * nk2);
+  const uint32_t* key_lo = reinterpret_cast<const uint32_t*>(d.cell_key);
+  uint32_t kv[RW_CELLS / 64];
+#pragma unroll
+  for (int k = 0; k < RW_CELLS / 64; ++k) {
+    const int i = lane + 64 * k;
+    kv[k] = 0u;
+    if (64 * k < pad && i < ncz) kv[k] = key_lo[2 * (int64_t)(cmin + i)];
+  }
+#pragma unroll
+  for (int k = 0; k < RW_CELLS / 64; ++k) {
+    const int i = lane + 64 * k;
+    if (64 * k < pad && i < pad) ck[i] = i < ncz ? kv[k] - kbase : 0xffffffffu;
+  }
   wave_sync();
   // keys of (line of j, zlo), relative to the span's first line
   const int32_t zlo = z >= 2 ? z - 2 : 0, zhi = z + 2 < nk2 ? z + 2 : nk2 - 1;
@@ -986,6 +1009,9 @@ __device__ __forceinline__ bool rows_staged(const Dev& d, int64_t tile, int g, i
     for (int j = 0; j < 5; ++j) pos[j] = v[j] < klo[j] ? pos[j] + h : pos[j];
   }
   const uint32_t dz = (uint32_t)(zhi - zlo);
+  int rrow[5];  // read before the first store
+#pragma unroll
+  for (int j = 0; j < 5; ++j) rrow[j] = kRowOfOff3[g][j];
 #pragma unroll
   for (int j = 0; j < 5; ++j) {
     // the run: keys up to (line, zhi), at most five after pos
@@ -998,7 +1024,7 @@ __device__ __forceinline__ bool rows_staged(const Dev& d, int64_t tile, int g, i
     const int32_t cnt = e - pos[j];
     const bool ok = okx & (y + j - 2 >= 0) & (y + j - 2 < nk1);
     const int32_t packed = (ok && cnt > 0) ? ((cmin + pos[j] + 1) | (cnt << kRowCntShift)) : 0;
-    if (act) d.rows[(int64_t)kRowOfOff3[g][j] * d.rstride + c] = packed;
+    if (act) d.rows[(int64_t)rrow[j] * d.rstride + c] = packed;
   }
   return true;
 }
