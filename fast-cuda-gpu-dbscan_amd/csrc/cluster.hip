@@ -375,9 +375,10 @@ __device__ bool cells_joined(const Dev& d, const Par& P, int32_t qs, int32_t qe,
   constexpr uint32_t kFrac = (1u << kFixBits) - 1u;
   const int off[3] = {off0, D == 3 ? off1 : offl, offl};
   for (int32_t a = qs; a < qe; ++a) {
-    if (!allc && !d.core[a]) continue;
+    // the flag and the coordinates are loaded together
     uint32_t ag[D];
     load_g<D>(d, a, ag);
+    if (!allc && !d.core[a]) continue;
     float g2 = 0.f, h2 = 0.f;
 #pragma unroll
     for (int x = 0; x < D; ++x) {
@@ -402,11 +403,12 @@ __device__ bool cells_joined(const Dev& d, const Par& P, int32_t qs, int32_t qe,
       for (int j = 0; j < 4; ++j) {
         const bool in = q0 + j < b1;
         const int32_t q = in ? q0 + j : b0;
+        // (coordinates loaded with the flag, not after it)
         cf[j] = in && d.core[q];
 #pragma unroll
         for (int x = 0; x < D; ++x) {
-          qg[j][x] = (F32 && cf[j]) ? d.G[x][q] : 0u;
-          qx[j][x] = (!F32 && cf[j]) ? d.X[x][q] : 0.0;
+          qg[j][x] = F32 ? d.G[x][q] : 0u;
+          qx[j][x] = !F32 ? d.X[x][q] : 0.0;
         }
       }
 #pragma unroll
@@ -934,11 +936,12 @@ __device__ long long far_cell(const Dev& d, const Inst& I, const Par& P, int32_t
       const int o0 = D == 3 ? kRowOff3[r][0] : kRowOff2[r];
       const int o1 = D == 3 ? kRowOff3[r][1] : 0;
       const bool far_l = o0 == 2 || o0 == -2 || o1 == 2 || o1 == -2;
-      const int ol = d.cell_z[B] - zA;  // B lies in a neighbour line of A
+      // B's place along the line and its point range, loaded together
+      const int32_t zB = d.cell_z[B], b0 = d.cell_start[B], b1 = d.cell_start[B + 1];
+      const int ol = zB - zA;  // B lies in a neighbour line of A
       // in a near line only the cells two steps away along the line are far
       if (!far_l && ol != 2 && ol != -2) continue;
-      if (cells_joined<D, F32>(d, P, qs, qe, allc, d.cell_start[B], d.cell_start[B + 1], o0, o1,
-                               ol, t, freeb)) {
+      if (cells_joined<D, F32>(d, P, qs, qe, allc, b0, b1, o0, o1, ol, t, freeb)) {
         uf_union(d.parent, compA, d.cell_comp[B]);
         ++joins;
       }
@@ -1085,7 +1088,10 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
       }
       while (!got && seg < seg_end) {
         const int64_t i = seg + lane;
-        const bool cand = i < seg_end && d.cell_ncore[i] > 0 && d.cell_label[i] == kUnclaimed;
+        // both loads together (seg < seg_end, so seg is a cell)
+        const int64_t ic = i < seg_end ? i : seg;
+        const int32_t nco = d.cell_ncore[ic], lb = d.cell_label[ic];
+        const bool cand = i < seg_end && nco > 0 && lb == kUnclaimed;
         unsigned long long m = ballot(cand);
         while (m) {
           const int l = __ffsll((long long)m) - 1;
@@ -1143,9 +1149,13 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
     // then takes ceil(bits / (64 T)) steps instead of the most bits of one
     // lane / T
     {
+      // all nine loads in flight before the first LDS write
       const RowsOf rows = rows_of(d, A);
+      int32_t rv[9];
 #pragma unroll
-      for (int r = 0; r < 9; ++r) pst.rows[r][lane] = act ? rows[r] : 0;
+      for (int r = 0; r < 9; ++r) rv[r] = act ? rows[r] : 0;
+#pragma unroll
+      for (int r = 0; r < 9; ++r) pst.rows[r][lane] = rv[r];
     }
     const int nb = __popcll(m0);
     edges += nb;
@@ -1196,15 +1206,22 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
       int32_t lab[T];
 #pragma unroll
       for (int j = 0; j < T; ++j) lab[j] = (j < nsub && v[j]) ? d.cell_label[tg[j]] : ow[j];
-      bool cl[T];
+      // the claims are all issued before any result is used, so that they
+      // are in flight together
+      bool cl[T], need[T];
+      int32_t old[T];
 #pragma unroll
       for (int j = 0; j < T; ++j) {
-        cl[j] = false;
-        if (j < nsub && v[j] && lab[j] == kUnclaimed) {
-          const int32_t old = atomicCAS(&d.cell_label[tg[j]], kUnclaimed, ow[j]);
-          cl[j] = old == kUnclaimed;
-          lab[j] = cl[j] ? ow[j] : old;
-        }
+        need[j] = j < nsub && v[j] && lab[j] == kUnclaimed;
+        old[j] = lab[j];
+      }
+#pragma unroll
+      for (int j = 0; j < T; ++j)
+        if (need[j]) old[j] = atomicCAS(&d.cell_label[tg[j]], kUnclaimed, ow[j]);
+#pragma unroll
+      for (int j = 0; j < T; ++j) {
+        cl[j] = need[j] && old[j] == kUnclaimed;
+        lab[j] = cl[j] ? ow[j] : old[j];
       }
 #pragma unroll
       for (int j = 0; j < T; ++j) {
