@@ -495,12 +495,14 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
       t.cells += len;
       const int32_t st = pk & kRowCellMask;
       int32_t bnd[6], nco[5], zb[5];
+      // unconditional loads at clamped indices (st + len <= C), so that all
+      // sixteen are in flight together; entries past the run are unused
 #pragma unroll
-      for (int k = 0; k < 6; ++k) bnd[k] = k <= len ? d.cell_start[st + k] : 0;
+      for (int k = 0; k < 6; ++k) bnd[k] = d.cell_start[k <= len ? st + k : st];
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
-        zb[k] = k < len ? d.cell_z[st + k] : 0;
-        nco[k] = k < len ? d.cell_ncore[st + k] : 0;
+        zb[k] = d.cell_z[k < len ? st + k : st];
+        nco[k] = d.cell_ncore[k < len ? st + k : st];
       }
       float g2 = 0.f, h2 = 0.f;  // over the line's axes
       {
@@ -538,25 +540,30 @@ __device__ void adj_lane(const Dev& d, const Inst& I, const Par& P, int32_t p, i
       for (int32_t t0 = 0; t0 < s.tot; t0 += B) {
         int32_t qi[B];
         int kk[B];
-        bool cf[B];
+        bool cf[B], need[B];
+        uint8_t cb[B];
         uint32_t cg[B][D];
         double cx[B][D];
+        // flags and coordinates of the batch loaded together, unconditionally
+        // (qi is a point either way)
 #pragma unroll
         for (int j = 0; j < B; ++j) {
           int jr = 0;
           const bool in = t0 + j < s.tot;
           qi[j] = in ? s.at(t0 + j, jr) : p;
           kk[j] = jr;
-          const bool need = in && !((m0 >> (base + jr)) & 1ull);
-          cf[j] = need && d.core[qi[j]];
+          need[j] = in && !((m0 >> (base + jr)) & 1ull);
+          cb[j] = d.core[qi[j]];
 #pragma unroll
           for (int k = 0; k < D; ++k) {
             if (F32)
-              cg[j][k] = need ? d.G[k][qi[j]] : 0u;
+              cg[j][k] = d.G[k][qi[j]];
             else
-              cx[j][k] = need ? d.X[k][qi[j]] : 0.0;
+              cx[j][k] = d.X[k][qi[j]];
           }
         }
+#pragma unroll
+        for (int j = 0; j < B; ++j) cf[j] = need[j] && cb[j];
 #pragma unroll
         for (int j = 0; j < B; ++j) {
           if (cf[j] && !((m0 >> (base + kk[j])) & 1ull)) {
