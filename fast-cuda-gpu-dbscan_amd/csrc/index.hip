@@ -423,7 +423,14 @@ __global__ __launch_bounds__(256) void k_rs_total(Dev d, int pass, int nblk) {
   if (!pass_active(d, pass)) return;
   const uint32_t* row = d.hist + (int64_t)blockIdx.x * nblk;
   uint32_t s = 0;
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += row[b];
+  // eight loads in flight per thread (clamped indices, the excess not added)
+  for (int b0 = threadIdx.x; b0 < nblk; b0 += 8 * 256) {
+    uint32_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = row[b0 + u * 256 < nblk ? b0 + u * 256 : nblk - 1];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += b0 + u * 256 < nblk ? v[u] : 0u;
+  }
   s = wave_sum(s);
   __shared__ uint32_t red[4];
   if (lane_id() == 0) red[threadIdx.x >> 6] = s;
@@ -457,9 +464,23 @@ __global__ __launch_bounds__(256) void k_rs_rowscan(Dev d, int pass, int nblk) {
   if (!active || s_skip) return;
   const uint32_t* row = d.hist + (int64_t)blockIdx.x * nblk;
   uint32_t* out = d.offs + (int64_t)blockIdx.x * nblk;
+  // eight chunks' loads in flight per thread, then the chunks scanned in turn
+  constexpr int RU = 8;
+  uint32_t xv[RU];
   for (int b0 = 0; b0 < nblk; b0 += blockDim.x) {
+    const int u = (b0 / 256) % RU;
+    if (u == 0) {
+#pragma unroll
+      for (int q = 0; q < RU; ++q) {
+        const int bq = b0 + q * 256 + threadIdx.x;
+        xv[q] = row[bq < nblk ? bq : nblk - 1];
+      }
+    }
     int b = b0 + threadIdx.x;
-    uint32_t x = b < nblk ? row[b] : 0;
+    uint32_t x = 0;
+#pragma unroll
+    for (int q = 0; q < RU; ++q) x = q == u ? xv[q] : x;
+    x = b < nblk ? x : 0u;
     uint32_t inc = wave_incl_scan(x);
     if (lane_id() == 63) s_w[threadIdx.x >> 6] = inc;
     __syncthreads();
@@ -675,6 +696,25 @@ __global__ __launch_bounds__(CS_T) void k_csr_write(Dev d) {
   const int wv = threadIdx.x >> 6;
   uint64_t kr[CS_I], kp[CS_I];
   load_key_pairs<CS_I>(d, key, t0, kr, kp);
+  // every round's instance and packed cell coordinates, read from the
+  // instance table before the first store (the stores could alias it as far
+  // as the compiler knows, which would hold each round's reads behind the
+  // previous round's stores)
+  int32_t ir[CS_I], zr[CS_I];
+  uint64_t pcr[CS_I];
+#pragma unroll
+  for (int r = 0; r < CS_I; ++r) {
+    ir[r] = 0;
+    zr[r] = 0;
+    pcr[r] = 0;
+    const int64_t i = t0 + r * CS_T + threadIdx.x;
+    if (i < d.n && (i == 0 || kr[r] != kp[r])) {
+      ir[r] = inst_of_key(d, kr[r]);
+      const Inst& I = d.inst[ir[r]];
+      pcr[r] = pack_cell(I, kr[r]);
+      zr[r] = (int32_t)(I.dim == 3 ? pcr[r] >> 42 : pcr[r] >> 32);
+    }
+  }
   if (threadIdx.x == 0) s_run = d.scan_tmp[blockIdx.x];
   __syncthreads();
 #pragma unroll
@@ -692,13 +732,12 @@ __global__ __launch_bounds__(CS_T) void k_csr_write(Dev d) {
     if (inr) {
       d.cell_of[i] = (int32_t)cid;
       if (f) {
-        const int32_t inst = inst_of_key(d, k);
+        const uint64_t pc = pcr[r];
         d.cell_start[cid] = (int32_t)i;
         d.cell_key[cid] = k;
-        d.cell_inst[cid] = inst;
-        const uint64_t pc = pack_cell(d.inst[inst], k);
+        d.cell_inst[cid] = ir[r];
         d.cell_xyz[cid] = pc;
-        d.cell_z[cid] = (int32_t)(d.inst[inst].dim == 3 ? pc >> 42 : pc >> 32);
+        d.cell_z[cid] = zr[r];
       }
       if (i == d.n - 1) {
         d.cell_start[cid + 1] = (int32_t)d.n;
