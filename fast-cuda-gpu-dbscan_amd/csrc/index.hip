@@ -994,7 +994,7 @@ __device__ __forceinline__ bool rows_staged(const Dev& d, int64_t tile, int g, i
   for (int k = 0; k < RW_LINES / 64; ++k) {
     const int i = lane + 64 * k;
     ev[k] = make_int2(-1, -1);
-    if (64 * k < span && i < span && gl + i < NL) ev[k] = d.lines[gl + i];
+    if (i < span && gl + i < NL) ev[k] = d.lines[gl + i];
   }
 #pragma unroll
   for (int k = 0; k < RW_LINES / 64; ++k) {
@@ -1022,12 +1022,12 @@ __device__ __forceinline__ bool rows_staged(const Dev& d, int64_t tile, int g, i
   for (int k = 0; k < RW_CELLS / 64; ++k) {
     const int i = lane + 64 * k;
     kv[k] = 0u;
-    if (64 * k < pad && i < ncz) kv[k] = key_lo[2 * (int64_t)(cmin + i)];
+    if (i < ncz) kv[k] = key_lo[2 * (int64_t)(cmin + i)];
   }
 #pragma unroll
   for (int k = 0; k < RW_CELLS / 64; ++k) {
     const int i = lane + 64 * k;
-    if (64 * k < pad && i < pad) ck[i] = i < ncz ? kv[k] - kbase : 0xffffffffu;
+    if (i < pad) ck[i] = i < ncz ? kv[k] - kbase : 0xffffffffu;
   }
   wave_sync();
   // keys of (line of j, zlo), relative to the span's first line
