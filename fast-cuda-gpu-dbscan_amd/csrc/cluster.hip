@@ -1059,11 +1059,14 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
   wave_sync();
   const unsigned long long t_begin = __builtin_amdgcn_s_memrealtime();
   // the seeding cursor over this wave's segment of cells persists across passes
-  int64_t seg = d.seg_pos[gw];
-  const int64_t seg_end = (int64_t)C * (gw + 1) / W;
+  // (cell indices, below 2^31; the per-pass counters are bounded by the
+  // pop budget: 32-bit, which keeps the wave-uniform state in fewer
+  // scalar registers)
+  int32_t seg = (int32_t)d.seg_pos[gw];
+  const int32_t seg_end = (int32_t)((int64_t)C * (gw + 1) / W);
   int left = budget;
   int sl = 0;  // seed-list length (wave-uniform)
-  long long pops = 0, ovf = 0, edges = 0, seeds = 0, rounds = 0;
+  int pops = 0, ovf = 0, edges = 0, seeds = 0, rounds = 0;
   int32_t cur = -1;    // chain of the cells in the list when not mixed
   bool mixed = false;  // the list holds queue cells (chains read per cell)
   for (;;) {
@@ -1094,9 +1097,9 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
         }
       }
       while (!got && seg < seg_end) {
-        const int64_t i = seg + lane;
+        const int32_t i = seg + lane;
         // both loads together (seg < seg_end, so seg is a cell)
-        const int64_t ic = i < seg_end ? i : seg;
+        const int32_t ic = i < seg_end ? i : seg;
         const int32_t nco = d.cell_ncore[ic], lb = d.cell_label[ic];
         const bool cand = i < seg_end && nco > 0 && lb == kUnclaimed;
         unsigned long long m = ballot(cand);

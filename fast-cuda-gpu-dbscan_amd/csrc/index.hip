@@ -938,7 +938,7 @@ __device__ __forceinline__ void write_rows(const Dev& d, int32_t c, bool act, in
 // cells do not fit the LDS slots (one that crosses an axis-0 boundary,
 // long or sparse lines, two instances of a batch, no line table, 2-D) is
 // listed for k_rows_global.
-constexpr int RW_LINES = 256;  // line-table entries of a tile's span (at most)
+constexpr int RW_LINES = 512;  // line-table entries of a tile's span (at most)
 constexpr int RW_CELLS = 512;  // cell keys staged per wavefront (a power of two)
 
 // Rows of one (tile, line group) from LDS; false when the tile does not fit
