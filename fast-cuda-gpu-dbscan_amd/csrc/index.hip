@@ -19,6 +19,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 
 namespace dclust {
@@ -317,13 +318,26 @@ __global__ void k_grid_setup(Dev d) {
 }
 
 // --------------------------------------------------------------- cellkey --
+// Sort keys are 32-bit when the key space of the call fits (at most four
+// 8-bit passes), else 64-bit; the key buffers are viewed as either.  The
+// host picks the width from the key bits it knows (the grid set-up's, or
+// the previous run's for a captured graph: a replay whose keys are wider
+// than its passes cover is abandoned by k_csr_count, so a truncated key is
+// never used).
+template <bool K32>
+using SortKey = typename std::conditional<K32, uint32_t, uint64_t>::type;
+template <bool K32>
+__device__ __forceinline__ SortKey<K32>* key_buf(const Dev& d, int b) {
+  return reinterpret_cast<SortKey<K32>*>(d.key[b]);
+}
+
 __device__ __forceinline__ int32_t cell_index(double x, const Inst& I, int k) {
   double t = floor(cell_t(x, I.mn[k], I.inv_w));
   int32_t c = (t >= 0.0) ? (int32_t)fmin(t, (double)(I.nk[k] - 1)) : 0;
   return c;
 }
 
-template <int D>
+template <int D, bool K32>
 __device__ __forceinline__ void cellkey_points(const Dev& d, const Inst& I, const Chunk& ch) {
   const double* p0 = d.xyz + I.xyz_off + (ch.begin - I.pt_begin) * D;
   const int64_t np = ch.end - ch.begin;
@@ -342,22 +356,23 @@ __device__ __forceinline__ void cellkey_points(const Dev& d, const Inst& I, cons
         int64_t lin = cell_index(v[u][0], I, 0);
 #pragma unroll
         for (int k = 1; k < D; ++k) lin = lin * I.nk[k] + cell_index(v[u][k], I, k);
-        d.key[0][ch.begin + i] = (uint64_t)(I.key_base + lin);
+        key_buf<K32>(d, 0)[ch.begin + i] = (SortKey<K32>)(I.key_base + lin);
         d.val[0][ch.begin + i] = (int32_t)(ch.begin + i);
       }
     }
   }
 }
 
+template <bool K32>
 __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
   const Chunk ch = d.chunks[blockIdx.x];
   // a copy: the key stores could alias the instance table as far as the
   // compiler knows, which would reload its fields after every store
   const Inst I = d.inst[ch.inst];
   if (I.dim == 3)
-    cellkey_points<3>(d, I, ch);
+    cellkey_points<3, K32>(d, I, ch);
   else
-    cellkey_points<2>(d, I, ch);
+    cellkey_points<2, K32>(d, I, ch);
 }
 
 // ------------------------------------------------------------ radix sort --
@@ -391,9 +406,10 @@ __device__ __forceinline__ bool pass_active(const Dev& d, int pass) {
   return pass * 8 < (int)d.ctl[C_KEYBITS];
 }
 
+template <bool K32>
 __global__ __launch_bounds__(RS_T) void k_rs_hist(Dev d, int pass, int nblk) {
   if (!pass_active(d, pass)) return;
-  const uint64_t* keys = d.key[sort_parity(d, pass)];
+  const SortKey<K32>* keys = key_buf<K32>(d, sort_parity(d, pass));
   const int shift = pass * 8;
   const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
   // all of the thread's keys are loaded at once (an index past the end reads
@@ -497,15 +513,17 @@ __global__ __launch_bounds__(256) void k_rs_rowscan(Dev d, int pass, int nblk) {
 // per round, waves in order, lanes in order) so equal digits keep their
 // input order; the tile is staged in LDS in local digit order and written
 // out coalesced per digit run.
+template <bool K32>
 __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) {
+  using KT = SortKey<K32>;
   if (!pass_active(d, pass) || !d.pass_scatter[pass]) return;
   const int par = sort_parity(d, pass);
-  const uint64_t* kin = d.key[par];
+  const KT* kin = key_buf<K32>(d, par);
   const int32_t* vin = d.val[par];
-  uint64_t* kout = d.key[par ^ 1];
+  KT* kout = key_buf<K32>(d, par ^ 1);
   int32_t* vout = d.val[par ^ 1];
   const int shift = pass * 8;
-  __shared__ uint64_t s_k[RS_TILE];
+  __shared__ KT s_k[RS_TILE];
   __shared__ int32_t s_v[RS_TILE];
   __shared__ uint32_t s_lstart[256], s_run[256], s_goff[256];
   __shared__ uint32_t s_wc[4][256];
@@ -514,7 +532,7 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) 
   // the tile's keys and values are loaded at once, before the digit
   // offsets, so that their latencies overlap (an index past the end reads
   // element 0, which is not ranked)
-  uint64_t kr[RS_I];
+  KT kr[RS_I];
   int32_t vr[RS_I];
 #pragma unroll
   for (int r = 0; r < RS_I; ++r) {
@@ -542,7 +560,7 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) 
 #pragma unroll
   for (int r = 0; r < RS_I; ++r) {
     const bool v = t0 + r * RS_T + tid < d.n;
-    const uint64_t key = kr[r];
+    const KT key = kr[r];
     const int32_t val = vr[r];
     uint32_t dg = (uint32_t)(key >> shift) & 255u;
     unsigned long long peers = match8(dg, v);
@@ -566,7 +584,7 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) 
   }
   const int64_t tn = (d.n - t0) < RS_TILE ? (d.n - t0) : RS_TILE;
   for (int l = tid; l < tn; l += RS_T) {
-    uint64_t key = s_k[l];
+    const KT key = s_k[l];
     uint32_t dg = (uint32_t)(key >> shift) & 255u;
     int64_t g = (int64_t)s_goff[dg] + (l - s_lstart[dg]);
     kout[g] = key;
@@ -582,15 +600,16 @@ constexpr int CS_T = 256;
 constexpr int CS_I = 8;
 constexpr int CS_TILE = CS_T * CS_I;
 
-__device__ __forceinline__ const uint64_t* sorted_keys(const Dev& d) {
-  return d.key[sort_parity(d, kMaxPasses)];
+template <bool K32>
+__device__ __forceinline__ const SortKey<K32>* sorted_keys(const Dev& d) {
+  return key_buf<K32>(d, sort_parity(d, kMaxPasses));
 }
 
 // Sorted keys i and i - 1 of each of a thread's rounds, loaded before any
 // is used (indices clamped into the array) so that the loads are in flight
 // together; key i starts a cell when i < n and it differs from key i - 1.
-template <int NI>
-__device__ __forceinline__ void load_key_pairs(const Dev& d, const uint64_t* key, int64_t t0,
+template <int NI, typename KT>
+__device__ __forceinline__ void load_key_pairs(const Dev& d, const KT* key, int64_t t0,
                                                uint64_t (&k)[NI], uint64_t (&kp)[NI]) {
 #pragma unroll
   for (int r = 0; r < NI; ++r) {
@@ -605,12 +624,13 @@ __device__ __forceinline__ void load_key_pairs(const Dev& d, const uint64_t* key
 // leaves the keys unsorted, so the run is abandoned: every later kernel
 // returns at once and the host runs the call again; counting unsorted keys
 // here is harmless)
+template <bool K32>
 __global__ __launch_bounds__(CS_T) void k_csr_count(Dev d, int passes) {
   if (blockIdx.x == 0 && threadIdx.x == 0 && passes < kMaxPasses &&
       d.ctl[C_KEYBITS] > 8 * passes)
     d.ctl[C_ABANDON] = 1;
   if (abandoned(d)) return;
-  const uint64_t* key = sorted_keys(d);
+  const auto* key = sorted_keys<K32>(d);
   const int64_t t0 = (int64_t)blockIdx.x * CS_TILE;
   uint64_t kr[CS_I], kp[CS_I];
   load_key_pairs<CS_I>(d, key, t0, kr, kp);
@@ -687,9 +707,10 @@ __device__ __forceinline__ void unpack_cell(int dim, uint64_t pc, int32_t* kc) {
   }
 }
 
+template <bool K32>
 __global__ __launch_bounds__(CS_T) void k_csr_write(Dev d) {
   if (abandoned(d)) return;
-  const uint64_t* key = sorted_keys(d);
+  const auto* key = sorted_keys<K32>(d);
   const int64_t t0 = (int64_t)blockIdx.x * CS_TILE;
   __shared__ int s_w[4];
   __shared__ int64_t s_run;
@@ -1208,19 +1229,29 @@ void launch_index_head(const Dev& d, int nchunks, hipStream_t s) {
   hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(1024), 0, s, d);
 }
 
-void launch_index_sort(const Dev& d, int64_t n, int nchunks, int passes, hipStream_t s) {
-  hipLaunchKernelGGL(k_cellkey, dim3(nchunks), dim3(256), 0, s, d);
+template <bool K32>
+static void index_sort(const Dev& d, int64_t n, int nchunks, int passes, hipStream_t s) {
+  hipLaunchKernelGGL(k_cellkey<K32>, dim3(nchunks), dim3(256), 0, s, d);
   const int nblk = (int)((n + RS_TILE - 1) / RS_TILE);
   for (int p = 0; p < passes; ++p) {
-    hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
+    hipLaunchKernelGGL(k_rs_hist<K32>, dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
     hipLaunchKernelGGL(k_rs_total, dim3(256), dim3(256), 0, s, d, p, nblk);
     hipLaunchKernelGGL(k_rs_rowscan, dim3(256), dim3(256), 0, s, d, p, nblk);
-    hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
+    hipLaunchKernelGGL(k_rs_scatter<K32>, dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
   }
   const int ncs = (int)((n + CS_TILE - 1) / CS_TILE);
-  hipLaunchKernelGGL(k_csr_count, dim3(ncs), dim3(CS_T), 0, s, d, passes);
+  hipLaunchKernelGGL(k_csr_count<K32>, dim3(ncs), dim3(CS_T), 0, s, d, passes);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, d.scan_tmp, ncs, (long long*)nullptr);
-  hipLaunchKernelGGL(k_csr_write, dim3(ncs), dim3(CS_T), 0, s, d);
+  hipLaunchKernelGGL(k_csr_write<K32>, dim3(ncs), dim3(CS_T), 0, s, d);
+}
+
+// `passes` radix passes: keys of at most 32 bits (four passes) are sorted as
+// 32-bit keys
+void launch_index_sort(const Dev& d, int64_t n, int nchunks, int passes, hipStream_t s) {
+  if (passes <= 4)
+    index_sort<true>(d, n, nchunks, passes, s);
+  else
+    index_sort<false>(d, n, nchunks, passes, s);
 }
 
 void launch_gather(const Dev& d, int64_t n, hipStream_t s) {
