@@ -353,7 +353,7 @@ hipError_t enqueue_index(dclust_handle* h, const Dev& d, int64_t n, int nchunks,
                          int passes, bool graph) {
   const hipStream_t s = h->stream;
   hipError_t e;
-  launch_index_sort(d, n, nchunks, passes, s);
+  launch_index_sort(d, n, nchunks, passes, h->num_cu, s);
   launch_gather(d, n, s);
   launch_lines(d, grid_cells, s);
   if ((e = mark(h, E_ROWS0, graph)) != hipSuccess) return e;

@@ -376,13 +376,17 @@ __global__ __launch_bounds__(256) void k_cellkey(Dev d) {
 }
 
 // ------------------------------------------------------------ radix sort --
-// LSD radix sort, 8-bit digits, 2048-key tiles (256 threads x 8 keys: the
-// scatter's 24 KiB of LDS lets six workgroups share a CU; 4096-key tiles
-// were slower, 1024-key tiles too).  Buffer parity: the number of scatters
-// executed so far.
+// LSD radix sort, 8-bit digits, 2048-key tiles (256 threads x 8 keys; with
+// 64-bit keys the scatter's 31 KiB of LDS lets five workgroups share a CU,
+// and 4096-key tiles were slower, 1024-key tiles too), or 4096-key tiles for
+// 32-bit keys where they need fewer rounds of workgroups (sort_items).
+// Buffer parity: the number of scatters executed so far.
 constexpr int RS_T = 256;
 constexpr int RS_I = 8;
 constexpr int RS_TILE = RS_T * RS_I;
+// Keys per thread: 8 (2048-key tiles), or 16 (4096-key tiles) for 32-bit
+// keys when that takes fewer rounds of resident workgroups (the histogram
+// buffers are sized for the smaller tiles; see sort_items).
 
 __device__ __forceinline__ int sort_parity(const Dev& d, int pass) {
   int p = 0;
@@ -406,17 +410,17 @@ __device__ __forceinline__ bool pass_active(const Dev& d, int pass) {
   return pass * 8 < (int)d.ctl[C_KEYBITS];
 }
 
-template <bool K32>
+template <bool K32, int NI>
 __global__ __launch_bounds__(RS_T) void k_rs_hist(Dev d, int pass, int nblk) {
   if (!pass_active(d, pass)) return;
   const SortKey<K32>* keys = key_buf<K32>(d, sort_parity(d, pass));
   const int shift = pass * 8;
-  const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
+  const int64_t t0 = (int64_t)blockIdx.x * (RS_T * NI);
   // all of the thread's keys are loaded at once (an index past the end reads
   // key 0 and is not counted), so their latencies overlap
-  uint32_t dg[RS_I];
+  uint32_t dg[NI];
 #pragma unroll
-  for (int r = 0; r < RS_I; ++r) {
+  for (int r = 0; r < NI; ++r) {
     const int64_t i = t0 + r * RS_T + threadIdx.x;
     dg[r] = (uint32_t)(keys[i < d.n ? i : 0] >> shift) & 255u;
   }
@@ -424,7 +428,7 @@ __global__ __launch_bounds__(RS_T) void k_rs_hist(Dev d, int pass, int nblk) {
   h[threadIdx.x] = 0;
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < RS_I; ++r) {
+  for (int r = 0; r < NI; ++r) {
     const bool v = t0 + r * RS_T + threadIdx.x < d.n;
     const unsigned long long peers = match8(dg[r], v);
     if (v && mbcnt(peers) == 0) atomicAdd(&h[dg[r]], (uint32_t)__popcll(peers));
@@ -513,9 +517,10 @@ __global__ __launch_bounds__(256) void k_rs_rowscan(Dev d, int pass, int nblk) {
 // per round, waves in order, lanes in order) so equal digits keep their
 // input order; the tile is staged in LDS in local digit order and written
 // out coalesced per digit run.
-template <bool K32>
+template <bool K32, int NI>
 __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) {
   using KT = SortKey<K32>;
+  constexpr int TILE = RS_T * NI;
   if (!pass_active(d, pass) || !d.pass_scatter[pass]) return;
   const int par = sort_parity(d, pass);
   const KT* kin = key_buf<K32>(d, par);
@@ -523,19 +528,19 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) 
   KT* kout = key_buf<K32>(d, par ^ 1);
   int32_t* vout = d.val[par ^ 1];
   const int shift = pass * 8;
-  __shared__ KT s_k[RS_TILE];
-  __shared__ int32_t s_v[RS_TILE];
+  __shared__ KT s_k[TILE];
+  __shared__ int32_t s_v[TILE];
   __shared__ uint32_t s_lstart[256], s_run[256], s_goff[256];
   __shared__ uint32_t s_wc[4][256];
   const int tid = threadIdx.x, wv = tid >> 6;
-  const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
+  const int64_t t0 = (int64_t)blockIdx.x * TILE;
   // the tile's keys and values are loaded at once, before the digit
   // offsets, so that their latencies overlap (an index past the end reads
   // element 0, which is not ranked)
-  KT kr[RS_I];
-  int32_t vr[RS_I];
+  KT kr[NI];
+  int32_t vr[NI];
 #pragma unroll
-  for (int r = 0; r < RS_I; ++r) {
+  for (int r = 0; r < NI; ++r) {
     const int64_t i = t0 + r * RS_T + tid;
     const int64_t ic = i < d.n ? i : 0;
     kr[r] = kin[ic];
@@ -558,7 +563,7 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) 
     __syncthreads();
   }
 #pragma unroll
-  for (int r = 0; r < RS_I; ++r) {
+  for (int r = 0; r < NI; ++r) {
     const bool v = t0 + r * RS_T + tid < d.n;
     const KT key = kr[r];
     const int32_t val = vr[r];
@@ -582,7 +587,7 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) 
     }
     __syncthreads();
   }
-  const int64_t tn = (d.n - t0) < RS_TILE ? (d.n - t0) : RS_TILE;
+  const int64_t tn = (d.n - t0) < TILE ? (d.n - t0) : TILE;
   for (int l = tid; l < tn; l += RS_T) {
     const KT key = s_k[l];
     uint32_t dg = (uint32_t)(key >> shift) & 255u;
@@ -1229,15 +1234,29 @@ void launch_index_head(const Dev& d, int nchunks, hipStream_t s) {
   hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(1024), 0, s, d);
 }
 
-template <bool K32>
+// Keys per thread of the radix passes over n keys.  A scatter workgroup of
+// 2048 32-bit keys (23 KiB of LDS) shares a CU with five others, one of
+// 4096 keys (39 KiB) with three; a 4096-key tile takes about 1.6x as long.
+// The larger tiles are taken when their rounds of resident workgroups cost
+// less: config 3's 4 M keys run as 977 tiles in one round instead of 1954
+// tiles in two; 1 M keys (one partial round either way) and 32 M keys stay
+// with the smaller tiles.
+int sort_items(bool k32, int64_t n, int cus) {
+  if (!k32 || cus <= 0) return RS_I;
+  const int64_t r2 = (n + 2048 * 6LL * cus - 1) / (2048 * 6LL * cus);
+  const int64_t r4 = (n + 4096 * 4LL * cus - 1) / (4096 * 4LL * cus);
+  return 16 * r4 < 10 * r2 ? 16 : RS_I;
+}
+
+template <bool K32, int NI>
 static void index_sort(const Dev& d, int64_t n, int nchunks, int passes, hipStream_t s) {
   hipLaunchKernelGGL(k_cellkey<K32>, dim3(nchunks), dim3(256), 0, s, d);
-  const int nblk = (int)((n + RS_TILE - 1) / RS_TILE);
+  const int nblk = (int)((n + RS_T * NI - 1) / (RS_T * NI));
   for (int p = 0; p < passes; ++p) {
-    hipLaunchKernelGGL(k_rs_hist<K32>, dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
+    hipLaunchKernelGGL((k_rs_hist<K32, NI>), dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
     hipLaunchKernelGGL(k_rs_total, dim3(256), dim3(256), 0, s, d, p, nblk);
     hipLaunchKernelGGL(k_rs_rowscan, dim3(256), dim3(256), 0, s, d, p, nblk);
-    hipLaunchKernelGGL(k_rs_scatter<K32>, dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
+    hipLaunchKernelGGL((k_rs_scatter<K32, NI>), dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
   }
   const int ncs = (int)((n + CS_TILE - 1) / CS_TILE);
   hipLaunchKernelGGL(k_csr_count<K32>, dim3(ncs), dim3(CS_T), 0, s, d, passes);
@@ -1247,11 +1266,13 @@ static void index_sort(const Dev& d, int64_t n, int nchunks, int passes, hipStre
 
 // `passes` radix passes: keys of at most 32 bits (four passes) are sorted as
 // 32-bit keys
-void launch_index_sort(const Dev& d, int64_t n, int nchunks, int passes, hipStream_t s) {
-  if (passes <= 4)
-    index_sort<true>(d, n, nchunks, passes, s);
+void launch_index_sort(const Dev& d, int64_t n, int nchunks, int passes, int cus, hipStream_t s) {
+  if (passes > 4)
+    index_sort<false, RS_I>(d, n, nchunks, passes, s);
+  else if (sort_items(true, n, cus) == 16)
+    index_sort<true, 16>(d, n, nchunks, passes, s);
   else
-    index_sort<false>(d, n, nchunks, passes, s);
+    index_sort<true, RS_I>(d, n, nchunks, passes, s);
 }
 
 void launch_gather(const Dev& d, int64_t n, hipStream_t s) {
@@ -1264,13 +1285,6 @@ void launch_gather(const Dev& d, int64_t n, hipStream_t s) {
 void launch_lines(const Dev& d, int grid_cells, hipStream_t s) {
   hipLaunchKernelGGL(k_line_clear, dim3(grid_cells), dim3(256), 0, s, d);
   hipLaunchKernelGGL(k_line_mark, dim3(grid_cells), dim3(256), 0, s, d);
-}
-
-void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
-                       int passes, hipStream_t s) {
-  launch_index_sort(d, n, nchunks, passes, s);
-  launch_gather(d, n, s);
-  launch_lines(d, grid_cells, s);
 }
 
 // Row pass grid: one wavefront per (tile of 64 cells, line group) for up

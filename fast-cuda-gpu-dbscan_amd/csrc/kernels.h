@@ -11,12 +11,10 @@ void launch_init_head(const Dev& d, hipStream_t s);
 void launch_init_run(const Dev& d, int64_t waves, hipStream_t s);
 // bbox + grid set-up (per-instance cell width, cells per axis, key space)
 void launch_index_head(const Dev& d, int nchunks, hipStream_t s);
-// cell keys, `passes` radix passes, CSR compaction, gather, line table
-void launch_index_tail(const Dev& d, int64_t n, int nchunks, int grid_cells,
-                       int passes, hipStream_t s);
-// the same in three parts: keys, sort and CSR; the gather of the
-// coordinates; the line table (the last two are independent)
-void launch_index_sort(const Dev& d, int64_t n, int nchunks, int passes, hipStream_t s);
+// the index after the grid set-up, in three parts: cell keys, `passes`
+// radix passes and CSR compaction (tile size from the CU count `cus`); the
+// gather of the coordinates; the line table (the last two are independent)
+void launch_index_sort(const Dev& d, int64_t n, int nchunks, int passes, int cus, hipStream_t s);
 void launch_gather(const Dev& d, int64_t n, hipStream_t s);
 void launch_lines(const Dev& d, int grid_cells, hipStream_t s);
 // neighbour rows of every cell (k_rows)
