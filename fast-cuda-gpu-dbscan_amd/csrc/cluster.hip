@@ -63,7 +63,7 @@ constexpr int kQCache = 256;  // query points a heavy-cell wavefront keeps in LD
 
 template <bool F32>
 struct Batch {
-  static constexpr int value = F32 ? 6 : 4;  // candidates loaded per lane step
+  static constexpr int value = 4;  // candidates loaded per lane step
 };
 
 struct Tally {  // per-lane event counts (summed per wave by stat_add)
@@ -275,8 +275,11 @@ __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int3
   t.eval += ev;
 }
 
+// 6 waves per SIMD for the FP32 form (80 VGPRs with a batch of four
+// candidates; latency bound, so residency pays more than a longer batch), 5
+// for the FP64 form, whose coordinates need more registers
 template <bool F32>
-__global__ __launch_bounds__(256, 5) void k_core(Dev d) {
+__global__ __launch_bounds__(256, F32 ? 6 : 5) void k_core(Dev d) {
   if (abandoned(d)) return;
   const int64_t pl = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const bool act = pl < d.n;
