@@ -391,9 +391,10 @@ def batched_leg(hip, h, steps, warmup, check_oracle=True, count=4096, online_tun
 
 
 def _short(name):
-    """Kernel name without namespace, template and arguments."""
-    n = name.split("(")[0].split(" ")[-1]
-    return n.split("::")[-1].split("<")[0]
+    """Kernel name without namespace, template and arguments (template
+    arguments are dropped first: they may hold spaces)."""
+    n = name.split("(")[0].split("<")[0].split(" ")[-1]
+    return n.split("::")[-1]
 
 
 def summarize_profile(path, skip_runs, traffic=None):

@@ -450,6 +450,30 @@ def test_graph_replay_reruns_when_the_key_widens(handle, prec, monkeypatch):
     assert out[1][1] <= 8 and out[2][1] > 16
 
 
+@pytest.mark.parametrize("prec", PRECS)
+def test_graph_replay_reruns_when_the_key_passes_32_bits(handle, prec, monkeypatch):
+    # keys of at most 32 bits are sorted as 32-bit keys; a replay of such a
+    # graph whose cell keys need more bits is abandoned before its truncated
+    # keys are used, and the call runs eagerly with 64-bit keys; a graph
+    # captured for 64-bit keys then replays the narrower input too
+    monkeypatch.delenv("DCLUST_GRAPH", raising=False)
+    rng = np.random.default_rng(46)
+    n, eps, mp = 4000, 0.001, 4
+    centres = rng.uniform(0.05, 0.45, (8, 3))
+    narrow = centres[rng.integers(0, 8, n)] + rng.normal(0, 0.002, (n, 3))
+    narrow[0] = [0.0, 0.0, 0.0]
+    narrow[1] = [0.5, 0.5, 0.5]  # extent 0.5: 1000 cells per axis, 30-bit keys
+    wide = narrow.copy()
+    wide[1] = [1.0, 1.0, 1.0]  # extent 1.0: 2000 cells per axis, 33-bit keys
+    _check(handle, narrow[: n // 2], eps, mp, prec)
+    out = []
+    for x in [narrow, narrow, wide, wide, narrow]:
+        st = _check(handle, x, eps, mp, prec)
+        out.append((st["diag"][14], st["key_bits"]))
+    assert [r for r, _ in out] == [0, 1, 0, 1, 1]
+    assert out[0][1] <= 32 < out[2][1]
+
+
 def test_point_count_limit(handle, devmem):
     # cell indices are packed into 28 bits: 2^28 points are refused on the
     # host, before any buffer is touched
