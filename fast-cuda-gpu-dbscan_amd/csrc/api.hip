@@ -115,7 +115,7 @@ enum Slot {
   S_DTOT, S_PASS, S_CSTART, S_CKEY, S_CINST, S_CM, S_ROWS, S_CELLOF, S_X0, S_X1, S_X2,
   S_G0, S_G1, S_G2, S_ORIG, S_CORE, S_LABEL, S_PARENT, S_COMPMIN, S_PAIRS, S_Q0, S_Q1,
   S_KEYV, S_FLAG, S_RANK, S_SCAN, S_LABOUT, S_COREOUT, S_PAIRCNT, S_SEGPOS, S_LINES, S_AMASK,
-  S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP, S_ADJPTS, S_CORECELLS, S_TUNELAB, S_STAT, S_GLINES,
+  S_HEAVY, S_REC, S_RECN, S_CXYZ, S_CCOMP, S_ADJPTS, S_BORDERPTS, S_CORECELLS, S_TUNELAB, S_STAT, S_GLINES,
   S_CZ, S_XOFF, S_OS_LABEL, S_OS_STATE, S_OS_SEEDS, S_OS_LEN, S_OS_CONTACT, S_OS_FIRST, S_ROWSFB
 };
 
@@ -679,6 +679,8 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     const int64_t cap_c = (cell_comp_blocks(ntot) + kStripes - 1) / kStripes * 1024;
     d.adj_pts = SList{M.get<int32_t>(S_ADJPTS, (size_t)cap_p * kStripes),
                       (unsigned long long*)(d.stat + (int64_t)C_NADJPTS * kStripes), cap_p};
+    d.border_pts = SList{M.get<int32_t>(S_BORDERPTS, (size_t)cap_p * kStripes),
+                         (unsigned long long*)(d.stat + (int64_t)C_NBORDERPTS * kStripes), cap_p};
     d.heavy = SList{M.get<int32_t>(S_HEAVY, (size_t)cap_p * kStripes),
                     (unsigned long long*)(d.stat + (int64_t)C_NHEAVY * kStripes), cap_p};
     d.core_cells = SList{M.get<int32_t>(S_CORECELLS, (size_t)cap_c * kStripes),

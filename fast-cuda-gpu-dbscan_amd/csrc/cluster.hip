@@ -307,6 +307,8 @@ __global__ __launch_bounds__(256, F32 ? 6 : 5) void k_core(Dev d) {
   if (act) d.core[p] = isc;
   // core points of light cells: the lanes of the near-adjacency pass
   block_append(isc && qe - qs <= kHeavy, p, d.adj_pts);
+  // non-core points: the lanes of the border-label pass
+  block_append(act && !isc, p, d.border_pts);
   // per-cell core counts: the lanes of one cell form a contiguous run
   const bool head = act && (p == qs || lane == 0);
   const unsigned long long mh = ballot(head), mc = ballot(isc);
@@ -1595,17 +1597,19 @@ __device__ int32_t border_lane(const Dev& d, const Par& P, int32_t p, int32_t c,
   return best;
 }
 
+// One lane per non-core point (the list k_core fills; a wave's points are
+// consecutive ones of one workgroup there, in cell order).
 template <bool F32>
 __global__ __launch_bounds__(256) void k_border(Dev d) {
   if (abandoned(d)) return;
-  const int64_t pl = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  bool act = pl < d.n;
-  const int32_t p = act ? (int32_t)pl : 0;
+  __shared__ long long s_pre[kStripes + 1];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i < slist_prefix(d.border_pts, s_pre);
+  const int32_t p = act ? slist_at(d.border_pts, s_pre, i) : 0;
   int32_t c = 0, qs = 0, qe = 0;
   int dim = 0;
   const Inst* I = d.inst;
   if (act) {
-    act = !d.core[p];
     c = d.cell_of[p];
     qs = d.cell_start[c];
     qe = d.cell_start[c + 1];

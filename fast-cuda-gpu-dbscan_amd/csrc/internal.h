@@ -155,9 +155,10 @@ enum Ctl : int {
   C_ADJ_LINES,    // k_adj: non-empty near lines examined
   C_ADJ_CELLS,    // k_adj: cells of those lines
   C_NROWSFB,      // row work items left to the global-memory pass (Dev::rows_fb)
-  C_NCTL = 48
+  C_NBORDERPTS,   // non-core points (listed in Dev::border_pts)
+  C_NCTL = 49
 };
-static_assert(C_NROWSFB < C_NCTL, "control block too small");
+static_assert(C_NBORDERPTS < C_NCTL, "control block too small");
 
 // Cell-key space: the sort key of a cell is its instance's key base plus its
 // linear index, and the radix sort orders keys of up to 63 bits, so the
@@ -233,6 +234,7 @@ struct Dev {
   int32_t* cell_comp;   // [N] component (root chain) of each core cell, -1 otherwise
   SList heavy;          // heavy cells (more than kHeavy points)
   SList adj_pts;        // core points of light cells (near-adjacency lanes)
+  SList border_pts;     // non-core points (border-label lanes)
   SList core_cells;     // cells with core points (far-pair lanes)
   int32_t* rows;        // [kMaxRows][rstride] packed neighbour lines (first cell, count),
                         // line-major so that consecutive cells are adjacent
