@@ -61,10 +61,7 @@ namespace dclust {
 
 constexpr int kQCache = 256;  // query points a heavy-cell wavefront keeps in LDS
 
-template <bool F32>
-struct Batch {
-  static constexpr int value = 4;  // candidates loaded per lane step
-};
+constexpr int kCoreBatch = 4;  // neighbour-count candidates loaded per lane step
 
 struct Tally {  // per-lane event counts (summed per wave by stat_add)
   int band = 0, flip = 0, asym = 0, eval = 0;
@@ -175,7 +172,7 @@ template <int D, bool F32>
 __device__ void core_lane(const Dev& d, const Par& P, int32_t p, int32_t c, int32_t qs,
                           int32_t qe, int R, int32_t* rk, Tally& t, bool& isc) {
   constexpr int NR = D == 3 ? 25 : 5;
-  constexpr int B = Batch<F32>::value;
+  constexpr int B = kCoreBatch;
   int cnt = qe - qs;  // the own cell is a clique, p itself included
   if (cnt >= P.minpts) {
     isc = true;
