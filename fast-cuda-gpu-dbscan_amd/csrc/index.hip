@@ -513,10 +513,13 @@ __global__ __launch_bounds__(256) void k_rs_rowscan(Dev d, int pass, int nblk) {
   }
 }
 
-// Stable scatter: ranks are computed round by round (256 consecutive keys
-// per round, waves in order, lanes in order) so equal digits keep their
-// input order; the tile is staged in LDS in local digit order and written
-// out coalesced per digit run.
+// Stable scatter.  Each wave owns a contiguous run of 64 * NI keys of the
+// tile and ranks them round by round (64 consecutive keys per round, lanes in
+// order) against counters of its own in LDS, with no barrier between rounds
+// (a wave's LDS accesses complete in order); one barrier later, each digit's
+// counts of the waves before it and the tile's local starts turn the ranks
+// into positions, so equal digits keep their input order.  The tile is
+// staged in LDS in local digit order and written out coalesced per digit run.
 template <bool K32, int NI>
 __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) {
   using KT = SortKey<K32>;
@@ -530,70 +533,84 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) 
   const int shift = pass * 8;
   __shared__ KT s_k[TILE];
   __shared__ int32_t s_v[TILE];
-  __shared__ uint32_t s_lstart[256], s_run[256], s_goff[256];
+  __shared__ uint32_t s_lstart[256], s_goff[256];
   __shared__ uint32_t s_wc[4][256];
+  __shared__ uint32_t s_ws[4];
   const int tid = threadIdx.x, wv = tid >> 6;
   const int64_t t0 = (int64_t)blockIdx.x * TILE;
-  // the tile's keys and values are loaded at once, before the digit
+  const int64_t w0 = t0 + (int64_t)wv * (64 * NI) + lane_id();
+  // the wave's keys and values are loaded at once, before the digit
   // offsets, so that their latencies overlap (an index past the end reads
   // element 0, which is not ranked)
   KT kr[NI];
   int32_t vr[NI];
 #pragma unroll
   for (int r = 0; r < NI; ++r) {
-    const int64_t i = t0 + r * RS_T + tid;
+    const int64_t i = w0 + r * 64;
     const int64_t ic = i < d.n ? i : 0;
     kr[r] = kin[ic];
     vr[r] = vin[ic];
   }
+  const uint32_t c = d.hist[(int64_t)tid * nblk + blockIdx.x];
+  s_goff[tid] = d.offs[(int64_t)tid * nblk + blockIdx.x];
+#pragma unroll
+  for (int w = 0; w < 4; ++w) s_wc[w][tid] = 0;
+  // the tile's local digit starts: an exclusive scan of its digit counts
+  const uint32_t inc = wave_incl_scan(c);
+  if (lane_id() == 63) s_ws[wv] = inc;
+  __syncthreads();
+  uint32_t lst = inc - c;
+  for (int w = 0; w < wv; ++w) lst += s_ws[w];
+  s_lstart[tid] = lst;
+  uint32_t lr[NI];
+#pragma unroll
+  for (int r = 0; r < NI; ++r) {
+    const bool v = w0 + r * 64 < d.n;
+    const uint32_t dg = (uint32_t)(kr[r] >> shift) & 255u;
+    const unsigned long long peers = match8(dg, v);
+    const uint32_t rank = mbcnt(peers);
+    const uint32_t base = s_wc[wv][dg];
+    if (v && rank == 0) s_wc[wv][dg] = base + (uint32_t)__popcll(peers);
+    lr[r] = base + rank;
+  }
+  __syncthreads();
   {
-    uint32_t c = d.hist[(int64_t)tid * nblk + blockIdx.x];
-    s_goff[tid] = d.offs[(int64_t)tid * nblk + blockIdx.x];
-    s_run[tid] = 0;
-    for (int w = 0; w < 4; ++w) s_wc[w][tid] = 0;
-    s_lstart[tid] = c;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-      uint32_t u = tid >= o ? s_lstart[tid - o] : 0;
-      __syncthreads();
-      s_lstart[tid] += u;
-      __syncthreads();
+    // position of wave w's first key with digit tid
+    const uint32_t a0 = s_wc[0][tid], a1 = s_wc[1][tid], a2 = s_wc[2][tid];
+    s_wc[0][tid] = lst;
+    s_wc[1][tid] = lst + a0;
+    s_wc[2][tid] = lst + a0 + a1;
+    s_wc[3][tid] = lst + a0 + a1 + a2;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < NI; ++r) {
+    if (w0 + r * 64 < d.n) {
+      const uint32_t dg = (uint32_t)(kr[r] >> shift) & 255u;
+      const uint32_t lp = s_wc[wv][dg] + lr[r];
+      s_k[lp] = kr[r];
+      s_v[lp] = vr[r];
     }
-    s_lstart[tid] -= c;
-    __syncthreads();
+  }
+  __syncthreads();
+  // write-out: every slot is read (past the tile's end the reads are
+  // unused), the stores are guarded
+  const int64_t tn = (d.n - t0) < TILE ? (d.n - t0) : TILE;
+#pragma unroll
+  for (int r = 0; r < NI; ++r) {
+    const int l = r * RS_T + tid;
+    kr[r] = s_k[l];
+    vr[r] = s_v[l];
   }
 #pragma unroll
   for (int r = 0; r < NI; ++r) {
-    const bool v = t0 + r * RS_T + tid < d.n;
-    const KT key = kr[r];
-    const int32_t val = vr[r];
-    uint32_t dg = (uint32_t)(key >> shift) & 255u;
-    unsigned long long peers = match8(dg, v);
-    int rank = mbcnt(peers);
-    int cnt = __popcll(peers);
-    if (v && rank == 0) s_wc[wv][dg] = cnt;
-    __syncthreads();
-    if (v) {
-      uint32_t pre = s_run[dg];
-      for (int w = 0; w < wv; ++w) pre += s_wc[w][dg];
-      uint32_t lp = s_lstart[dg] + pre + rank;
-      s_k[lp] = key;
-      s_v[lp] = val;
+    const int l = r * RS_T + tid;
+    const uint32_t dg = (uint32_t)(kr[r] >> shift) & 255u;
+    const int64_t g = (int64_t)s_goff[dg] + (l - (int)s_lstart[dg]);
+    if (l < tn) {
+      kout[g] = kr[r];
+      vout[g] = vr[r];
     }
-    __syncthreads();
-    if (v && rank == 0) {
-      atomicAdd(&s_run[dg], (uint32_t)cnt);
-      s_wc[wv][dg] = 0;
-    }
-    __syncthreads();
-  }
-  const int64_t tn = (d.n - t0) < TILE ? (d.n - t0) : TILE;
-  for (int l = tid; l < tn; l += RS_T) {
-    const KT key = s_k[l];
-    uint32_t dg = (uint32_t)(key >> shift) & 255u;
-    int64_t g = (int64_t)s_goff[dg] + (l - s_lstart[dg]);
-    kout[g] = key;
-    vout[g] = s_v[l];
   }
 }
 
