@@ -72,6 +72,8 @@ struct dclust_handle {
   hipGraphExec_t gexec = nullptr;  // captured run
   std::vector<uint8_t> gsig;       // launch state it was captured for
   int gpasses = 0;                 // expansion passes it holds
+  bool gphases = false;            // it records every phase event
+  bool cap_phases = false;         // (set while a capture runs)
   bool graph_broken = false;       // capture failed on this device: eager only
   bool graph_off = false;          // eager only (while autotuning)
   uint8_t* scratch = nullptr;      // this device's scratch array (no-allocation runs)
@@ -306,6 +308,16 @@ void drop_graph(dclust_handle* h) {
   h->gpasses = 0;
 }
 
+// Phase times of a replay: each event recorded inside a graph is a node that
+// leaves the device idle for about 5.5 us, so a captured run holds only the
+// events of the total and of the neighbour-count kernel, unless the
+// environment variable DCLUST_PHASE_TIMES=1 asks for every phase.
+bool phase_times_requested() {
+  const char* e = std::getenv("DCLUST_PHASE_TIMES");
+  return e && e[0] == '1';
+}
+bool always_marked(int e) { return e == E_START || e == E_CORE0 || e == E_CORE || e == E_END; }
+
 // Everything a launch sequence depends on besides the coordinates: the
 // device state (sizes, buffer and input/output pointers, switches), the
 // launch shape, the precision form and the instance and chunk tables.
@@ -318,8 +330,9 @@ std::vector<uint8_t> launch_signature(const Dev& d, const RunShape& r, bool f32,
     v.insert(v.end(), b, b + nb);
   };
   put(&d, sizeof(d));
-  const int64_t sh[8] = {r.seeds, r.threads, r.blocks, r.waves, r.pair_per, r.budget,
-                         r.pair_cap, f32 ? 1 : 0};
+  const int64_t sh[9] = {r.seeds,    r.threads, r.blocks,  r.waves,
+                         r.pair_per, r.budget,  r.pair_cap, f32 ? 1 : 0,
+                         phase_times_requested() ? 1 : 0};
   put(sh, sizeof(sh));
   put(inst.data(), sizeof(Inst) * inst.size());
   put(chunks.data(), sizeof(Chunk) * chunks.size());
@@ -328,6 +341,7 @@ std::vector<uint8_t> launch_signature(const Dev& d, const RunShape& r, bool f32,
 
 // Phase event; inside a stream capture it becomes a node of the graph.
 hipError_t mark(dclust_handle* h, int e, bool graph, hipStream_t s = nullptr) {
+  if (graph && !h->cap_phases && !always_marked(e)) return hipSuccess;
   if (!s) s = h->stream;
   return graph ? hipEventRecordWithFlags(h->ev[e], s, hipEventRecordExternal)
                : hipEventRecord(h->ev[e], s);
@@ -428,6 +442,7 @@ hipError_t capture_run(dclust_handle* h, const Dev& d, const RunShape& shape, bo
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
+  h->cap_phases = phase_times_requested();
   hipError_t body = mark(h, E_START, true);
   if (body == hipSuccess) {
     enqueue_head(d, nchunks, s);
@@ -495,8 +510,11 @@ void finish_run(dclust_handle* h, const Dev& d, const RunShape& shape, long long
   h->last_orig = d.orig;
   h->last_core_out = d.core_out;
   if (!stats) return;
+  // a replay of a graph without the phase events reports those phases as 0
+  const bool all_ev = !replayed || h->gphases;
+  auto phase_ms = [&](int a, int b) { return all_ev ? (double)ev_ms(h, a, b) : 0.0; };
   stats->t_index_ms = ev_ms(h, E_START, E_CORE0);
-  stats->t_rows_ms = ev_ms(h, E_ROWS0, E_INDEX);
+  stats->t_rows_ms = phase_ms(E_ROWS0, E_INDEX);
   stats->n_core_cells = ctl_h[C_NCORECELLS];
   stats->n_adj_points = ctl_h[C_NADJPTS];
   stats->n_lines = ctl_h[C_NLINES];
@@ -505,12 +523,12 @@ void finish_run(dclust_handle* h, const Dev& d, const RunShape& shape, long long
   stats->work[2] = ctl_h[C_ADJ_LINES];
   stats->work[3] = ctl_h[C_ADJ_CELLS];
   stats->t_core_ms = ev_ms(h, E_CORE0, E_CORE);
-  stats->t_adj_ms = ev_ms(h, E_CORE, E_ADJ);
-  stats->t_expand_ms = ev_ms(h, E_ADJ, E_EXPAND);
-  stats->t_merge_ms = ev_ms(h, E_EXPAND, E_MERGE);
-  stats->t_far_ms = ev_ms(h, E_MERGE, E_FAR);
-  stats->t_border_ms = ev_ms(h, E_FAR, E_BORDER);
-  stats->t_relabel_ms = ev_ms(h, E_BORDER, E_END);
+  stats->t_adj_ms = phase_ms(E_CORE, E_ADJ);
+  stats->t_expand_ms = phase_ms(E_ADJ, E_EXPAND);
+  stats->t_merge_ms = phase_ms(E_EXPAND, E_MERGE);
+  stats->t_far_ms = phase_ms(E_MERGE, E_FAR);
+  stats->t_border_ms = phase_ms(E_FAR, E_BORDER);
+  stats->t_relabel_ms = phase_ms(E_BORDER, E_END);
   stats->t_total_ms = ev_ms(h, E_START, E_END);
   stats->key_bits = (int32_t)ctl_h[C_KEYBITS];
   stats->n_cells = ctl_h[C_NCELL];
@@ -533,7 +551,7 @@ void finish_run(dclust_handle* h, const Dev& d, const RunShape& shape, long long
   stats->diag[1] = ctl_h[C_DENSE];
   stats->diag[2] = ctl_h[C_NHEAVY];
   stats->diag[3] = ctl_h[C_EDGES];
-  stats->diag[4] = (int64_t)(1000.0 * ev_ms(h, E_ADJ, E_PASS0));
+  stats->diag[4] = (int64_t)(1000.0 * phase_ms(E_ADJ, E_PASS0));
   stats->diag[5] = ctl_h[C_BUSY_SUM];
   stats->diag[6] = ctl_h[C_BUSY_MAX];
   stats->diag[7] = shape.waves;
@@ -754,6 +772,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
         h->gexec = ge;
         h->gsig = sig;
         h->gpasses = gp;
+        h->gphases = h->cap_phases;
       } else {
         (void)hipGetLastError();
         h->graph_broken = true;  // capture unsupported here: stay eager

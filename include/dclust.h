@@ -80,7 +80,12 @@ typedef struct dclust_params {
 } dclust_params;
 
 /* Per-run diagnostics.  Times are HIP-event milliseconds on the handle's
- * stream (the reference's clock() buckets, main.cu:410-432). */
+ * stream (the reference's clock() buckets, main.cu:410-432).  A replay of a
+ * captured run records only the events of t_total_ms, t_index_ms and
+ * t_core_ms (each event inside a graph idles the device for a few
+ * microseconds); t_rows_ms, t_adj_ms, t_expand_ms, t_merge_ms, t_far_ms,
+ * t_border_ms, t_relabel_ms and diag[4] are then 0, unless the environment
+ * variable DCLUST_PHASE_TIMES=1 is set when the call is made. */
 typedef struct dclust_stats {
   double t_total_ms;   /* first kernel to labels resident on the device   */
   double t_index_ms;   /* bbox + cell keys + radix sort + CSR + rows        */
