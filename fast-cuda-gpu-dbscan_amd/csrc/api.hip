@@ -73,6 +73,10 @@ struct dclust_handle {
   std::vector<uint8_t> gsig;       // launch state it was captured for
   int gpasses = 0;                 // expansion passes it holds
   bool gphases = false;            // it records every phase event
+  bool greadback = false;          // it ends with the counters' copies to `pin`
+  // pinned host copy of the control counters ([C_NCTL]) and their stripes
+  // ([C_NCTL][kStripes]), allocated with the handle (null: pageable copies)
+  long long* pin = nullptr;
   bool cap_phases = false;         // (set while a capture runs)
   bool graph_broken = false;       // capture failed on this device: eager only
   bool graph_off = false;          // eager only (while autotuning)
@@ -431,6 +435,17 @@ hipError_t enqueue_tail(dclust_handle* h, const Dev& d, bool f32, int grid_cells
   return hipGetLastError();
 }
 
+// The control counters and their stripes into the handle's pinned buffer
+// (inside a capture these become the graph's last nodes).
+hipError_t enqueue_readback(dclust_handle* h, const Dev& d, hipStream_t s) {
+  hipError_t e = hipMemcpyAsync(h->pin, d.ctl, sizeof(long long) * C_NCTL,
+                                hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h->pin + C_NCTL, d.stat, sizeof(long long) * C_NCTL * kStripes,
+                       hipMemcpyDeviceToHost, s);
+  return e;
+}
+
 // A whole run with no host read in between, captured into a graph:
 // `sort_passes` radix passes (the key width of the previous run; a wider key
 // abandons the run on the device) and `passes` expansion passes (a pass with
@@ -457,6 +472,7 @@ hipError_t capture_run(dclust_handle* h, const Dev& d, const RunShape& shape, bo
   // once and the host runs the call eagerly
   if (body == hipSuccess) launch_expand_check(d, passes, s);
   if (body == hipSuccess) body = enqueue_tail(h, d, f32, grid_cells, true);
+  if (body == hipSuccess && h->pin) body = enqueue_readback(h, d, s);
   e = hipStreamEndCapture(s, &g);
   if (body != hipSuccess) e = body;
   if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
@@ -464,11 +480,21 @@ hipError_t capture_run(dclust_handle* h, const Dev& d, const RunShape& shape, bo
   return e;
 }
 
-// Control counters and their striped parts, after the stream drains.
-hipError_t read_counters(const Dev& d, hipStream_t s, long long* ctl_h,
-                         std::vector<long long>& stat_h) {
-  hipError_t e = hipMemcpyAsync(ctl_h, d.ctl, sizeof(long long) * C_NCTL,
-                                hipMemcpyDeviceToHost, s);
+// Control counters and their striped parts, after the stream drains
+// (`enqueued`: the copies to the pinned buffer are already on the stream).
+hipError_t read_counters(dclust_handle* h, const Dev& d, hipStream_t s, long long* ctl_h,
+                         std::vector<long long>& stat_h, bool enqueued = false) {
+  hipError_t e = hipSuccess;
+  if (h->pin) {
+    if (!enqueued) e = enqueue_readback(h, d, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) {
+      std::memcpy(ctl_h, h->pin, sizeof(long long) * C_NCTL);
+      std::memcpy(stat_h.data(), h->pin + C_NCTL, sizeof(long long) * C_NCTL * kStripes);
+    }
+    return e;
+  }
+  e = hipMemcpyAsync(ctl_h, d.ctl, sizeof(long long) * C_NCTL, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess)
     e = hipMemcpyAsync(stat_h.data(), d.stat, sizeof(long long) * stat_h.size(),
                        hipMemcpyDeviceToHost, s);
@@ -773,6 +799,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
         h->gsig = sig;
         h->gpasses = gp;
         h->gphases = h->cap_phases;
+        h->greadback = h->pin != nullptr;
       } else {
         (void)hipGetLastError();
         h->graph_broken = true;  // capture unsupported here: stay eager
@@ -809,7 +836,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     if (h->gexec && !fixed_mem) {
       HIPCHK(upload());
       HIPCHK(hipGraphLaunch(h->gexec, s));
-      HIPCHK(read_counters(d, s, ctl_h, stat_h));
+      HIPCHK(read_counters(h, d, s, ctl_h, stat_h, h->greadback));
       if (!ctl_h[C_ABANDON]) {
         if (ctl_h[C_ERR]) {
           int first = 0;
@@ -879,7 +906,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     }
     ++npass;
     HIPCHK(enqueue_tail(h, d, f32, grid_cells, false));
-    HIPCHK(read_counters(d, s, ctl_h, stat_h));
+    HIPCHK(read_counters(h, d, s, ctl_h, stat_h));
     finish_run(h, d, shape, ctl_h, stat_h, npass, false, stats);
     if (npass >= kMaxExpandPasses) return DCLUST_E_CAPACITY;
     if (graph_on) {
@@ -1141,6 +1168,11 @@ int dclust_create(dclust_handle** out, int device) {
       dclust_destroy(h);
       return DCLUST_E_HIP;
     }
+  if (hipHostMalloc((void**)&h->pin, sizeof(long long) * C_NCTL * (1 + kStripes),
+                    hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    h->pin = nullptr;  // counters are then copied to pageable memory
+  }
   *out = h;
   return DCLUST_OK;
 }
@@ -1159,6 +1191,7 @@ void dclust_destroy(dclust_handle* h) {
     if (e) (void)hipEventDestroy(e);
   if (h->stream2) (void)hipStreamDestroy(h->stream2);
   if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->pin) (void)hipHostFree(h->pin);
   delete h;
 }
 
