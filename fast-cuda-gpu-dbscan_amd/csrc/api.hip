@@ -77,6 +77,7 @@ struct dclust_handle {
   // pinned host copy of the control counters ([C_NCTL]) and their stripes
   // ([C_NCTL][kStripes]), allocated with the handle (null: pageable copies)
   long long* pin = nullptr;
+  long long* pin_dev = nullptr;  // its device address
   bool cap_phases = false;         // (set while a capture runs)
   bool graph_broken = false;       // capture failed on this device: eager only
   bool graph_off = false;          // eager only (while autotuning)
@@ -436,14 +437,10 @@ hipError_t enqueue_tail(dclust_handle* h, const Dev& d, bool f32, int grid_cells
 }
 
 // The control counters and their stripes into the handle's pinned buffer
-// (inside a capture these become the graph's last nodes).
+// (inside a capture this becomes the graph's last kernel).
 hipError_t enqueue_readback(dclust_handle* h, const Dev& d, hipStream_t s) {
-  hipError_t e = hipMemcpyAsync(h->pin, d.ctl, sizeof(long long) * C_NCTL,
-                                hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(h->pin + C_NCTL, d.stat, sizeof(long long) * C_NCTL * kStripes,
-                       hipMemcpyDeviceToHost, s);
-  return e;
+  launch_readback(d, h->pin_dev, s);
+  return hipGetLastError();
 }
 
 // A whole run with no host read in between, captured into a graph:
@@ -1168,10 +1165,14 @@ int dclust_create(dclust_handle** out, int device) {
       dclust_destroy(h);
       return DCLUST_E_HIP;
     }
+  // coherent and mapped: the readback kernel's stores are visible to the
+  // host once the stream has drained
   if (hipHostMalloc((void**)&h->pin, sizeof(long long) * C_NCTL * (1 + kStripes),
-                    hipHostMallocDefault) != hipSuccess) {
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&h->pin_dev, h->pin, 0) != hipSuccess) {
     (void)hipGetLastError();
-    h->pin = nullptr;  // counters are then copied to pageable memory
+    if (h->pin) (void)hipHostFree(h->pin);
+    h->pin = h->pin_dev = nullptr;  // counters are then copied to pageable memory
   }
   *out = h;
   return DCLUST_OK;

@@ -61,6 +61,20 @@ void launch_init_head(const Dev& d, hipStream_t s) {
   hipLaunchKernelGGL(k_init_head, dim3(fill_grid(tot)), dim3(256), 0, s, d);
 }
 
+// End of a run: the control counters, then their stripes ([C_NCTL], then
+// [C_NCTL][kStripes]) into a host buffer (pinned and mapped), one workgroup.
+// Inside a captured run this follows the last kernel with no gap, where copy
+// nodes would each wait on a separate transfer.
+__global__ __launch_bounds__(1024) void k_readback(Dev d, long long* out) {
+  constexpr int tot = C_NCTL * (1 + kStripes);
+  for (int i = threadIdx.x; i < tot; i += blockDim.x)
+    out[i] = i < C_NCTL ? d.ctl[i] : d.stat[i - C_NCTL];
+}
+
+void launch_readback(const Dev& d, long long* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_readback, dim3(1), dim3(1024), 0, s, d, out);
+}
+
 // ------------------------------------------------------------------ bbox --
 
 // One block per chunk (chunks never straddle instances).  Block reduction,

@@ -6,6 +6,7 @@ namespace dclust {
 
 // start of a run: control counters, statistics, radix pass flags, bbox
 void launch_init_head(const Dev& d, hipStream_t s);
+void launch_readback(const Dev& d, long long* out, hipStream_t s);
 // per-run state of the clustering phases (labels, forest, minima, flags,
 // per-cell core counts, per-wave pair counts)
 void launch_init_run(const Dev& d, int64_t waves, hipStream_t s);

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+from collections.abc import Mapping
 
 import numpy as np
 
@@ -73,6 +74,33 @@ class Stats(C.Structure):
         out["diag"] = list(self.diag)
         out["work"] = list(self.work)
         return out
+
+
+_STATS_KEYS = tuple(k for k, _ in Stats._fields_)
+_STATS_KEYSET = frozenset(_STATS_KEYS)
+
+
+class StatsView(Mapping):
+    """One run's dclust_stats as a read-only mapping: a field is converted
+    when it is read (the arrays `diag` and `work` as lists), so a call in a
+    tight loop does not pay for building a dict of every field."""
+
+    __slots__ = ("_st",)
+
+    def __init__(self, st):
+        self._st = st
+
+    def __getitem__(self, k):
+        if k not in _STATS_KEYSET:
+            raise KeyError(k)
+        v = getattr(self._st, k)
+        return list(v) if k in ("diag", "work") else v
+
+    def __iter__(self):
+        return iter(_STATS_KEYS)
+
+    def __len__(self):
+        return len(_STATS_KEYS)
 
 
 class SynthInfo(C.Structure):
@@ -222,7 +250,7 @@ class Handle:
         _check(_lib().dclust_run_device(self._h, C.c_void_p(d_xyz), int(n), C.byref(params),
                                         C.c_void_p(d_labels), C.byref(st)),
                "dclust_run_device")
-        self.last_stats = st.as_dict()
+        self.last_stats = StatsView(st)
         return self.last_stats
 
     def run_device_noalloc(self, d_xyz: int, n: int, params: Params, d_labels: int):
