@@ -412,12 +412,14 @@ hipError_t enqueue_core(dclust_handle* h, const Dev& d, const RunShape& shape, b
   return hipSuccess;
 }
 
-void enqueue_pass(const Dev& d, const RunShape& shape, int pass, hipStream_t s) {
+// `check` > 0: the last pass of a captured run, whose carry kernel also
+// marks the run abandoned when work is still pending after it
+void enqueue_pass(const Dev& d, const RunShape& shape, int pass, hipStream_t s, int check = 0) {
   // (pass 0 needs no preparation: the counters it would reset are zero from
   // the start of the run; it drains no queue, so nothing is carried)
   if (pass > 0) launch_pass_prep(d, pass, s);
   launch_expand(d, pass, shape.blocks, shape.seeds, shape.threads, shape.budget, s);
-  if (pass > 0) launch_pass_carry(d, pass, s);
+  if (pass > 0 || check > 0) launch_pass_carry(d, pass, check, s);
 }
 
 // chain merge, far pairs, border points, canonical labels
@@ -461,13 +463,12 @@ hipError_t capture_run(dclust_handle* h, const Dev& d, const RunShape& shape, bo
     body = enqueue_index(h, d, d.n, nchunks, grid_cells, sort_passes, true);
   }
   if (body == hipSuccess) body = enqueue_core(h, d, shape, f32, true);
-  for (int p = 0; body == hipSuccess && p < passes; ++p) {
-    enqueue_pass(d, shape, p, s);
-    if (p == 0) body = mark(h, E_PASS0, true);
-  }
   // work still pending after the last pass: the rest of the graph returns at
   // once and the host runs the call eagerly
-  if (body == hipSuccess) launch_expand_check(d, passes, s);
+  for (int p = 0; body == hipSuccess && p < passes; ++p) {
+    enqueue_pass(d, shape, p, s, p == passes - 1 ? passes : 0);
+    if (p == 0) body = mark(h, E_PASS0, true);
+  }
   if (body == hipSuccess) body = enqueue_tail(h, d, f32, grid_cells, true);
   if (body == hipSuccess && h->pin) body = enqueue_readback(h, d, s);
   e = hipStreamEndCapture(s, &g);

@@ -1307,12 +1307,7 @@ __global__ __launch_bounds__(256) void k_expand(Dev d, int pass, int S, int budg
   stat_add(d, C_NCHAIN, lane == 0 ? seeds : 0);  // seeds is wave-uniform
 }
 
-// After the last of a fixed number of expansion passes: work still pending
-// (queue or seeding segments) marks the run as abandoned, and the kernels
-// after the expansion return at once (the host runs the call again).
-__global__ void k_expand_check(Dev d, int passes) {
-  if (d.ctl[C_Q0_N + (passes & 1)] != 0 || d.ctl[C_SEGS_LEFT] != 0) d.ctl[C_ABANDON] = 1;
-}
+
 
 // Resets the counters of the queue that pass `pass` fills and of the head
 // of the queue it drains, and marks the pass idle when nothing is pending.
@@ -1325,21 +1320,36 @@ __global__ void k_pass_prep(Dev d, int pass) {
 }
 
 // After a pass: queue entries no wave took (every wave spent its budget)
-// are appended to the queue of the next pass.  One block.
-__global__ __launch_bounds__(256) void k_pass_carry(Dev d, int pass) {
-  if (abandoned(d)) return;
-  if (pass == 0) return;
-  const long long qn = d.ctl[C_Q0_N + (pass & 1)];
-  const long long head = d.ctl[C_Q0_HEAD + (pass & 1)];
-  if (head >= qn) return;
+// are appended to the queue of the next pass.  One block.  With `check` =
+// the number of passes (after the last pass of a captured run), work still
+// pending afterwards (queue or seeding segments) marks the run as abandoned,
+// and the kernels after the expansion return at once (the host runs the call
+// again).
+__global__ __launch_bounds__(256) void k_pass_carry(Dev d, int pass, int check) {
   __shared__ long long s_base;
-  if (threadIdx.x == 0)
-    s_base = (long long)atomicAdd((unsigned long long*)&d.ctl[C_Q0_N + ((pass + 1) & 1)],
-                                  (unsigned long long)(qn - head));
-  __syncthreads();
-  const int32_t* qin = d.q[pass & 1];
-  int32_t* qout = d.q[(pass + 1) & 1];
-  for (long long i = head + threadIdx.x; i < qn; i += blockDim.x) qout[s_base + (i - head)] = qin[i];
+  long long qn = 0, head = 0;
+  if (!abandoned(d) && pass > 0) {
+    qn = d.ctl[C_Q0_N + (pass & 1)];
+    head = d.ctl[C_Q0_HEAD + (pass & 1)];
+  }
+  if (head < qn) {  // block-uniform: every thread read the same two counters
+    if (threadIdx.x == 0)
+      s_base = (long long)atomicAdd((unsigned long long*)&d.ctl[C_Q0_N + ((pass + 1) & 1)],
+                                    (unsigned long long)(qn - head));
+    __syncthreads();
+    const int32_t* qin = d.q[pass & 1];
+    int32_t* qout = d.q[(pass + 1) & 1];
+    for (long long i = head + threadIdx.x; i < qn; i += blockDim.x)
+      qout[s_base + (i - head)] = qin[i];
+  }
+  if (check > 0 && threadIdx.x == 0) {
+    // coherent loads: the add above went to L2, past this CU's cache
+    const long long left = __hip_atomic_load(&d.ctl[C_Q0_N + (check & 1)], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    const long long segs = __hip_atomic_load(&d.ctl[C_SEGS_LEFT], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    if (left != 0 || segs != 0) d.ctl[C_ABANDON] = 1;
+  }
 }
 
 // ----------------------------------------------------------------- merge --
@@ -1791,12 +1801,8 @@ void launch_pass_prep(const Dev& d, int pass, hipStream_t s) {
   hipLaunchKernelGGL(k_pass_prep, dim3(1), dim3(1), 0, s, d, pass);
 }
 
-void launch_pass_carry(const Dev& d, int pass, hipStream_t s) {
-  hipLaunchKernelGGL(k_pass_carry, dim3(1), dim3(256), 0, s, d, pass);
-}
-
-void launch_expand_check(const Dev& d, int passes, hipStream_t s) {
-  hipLaunchKernelGGL(k_expand_check, dim3(1), dim3(1), 0, s, d, passes);
+void launch_pass_carry(const Dev& d, int pass, int check, hipStream_t s) {
+  hipLaunchKernelGGL(k_pass_carry, dim3(1), dim3(256), 0, s, d, pass, check);
 }
 
 void launch_expand(const Dev& d, int pass, int grid, int seeds, int threads, int budget,

@@ -35,9 +35,8 @@ void launch_adj(const Dev& d, bool f32, int heavy_grid, hipStream_t s);
 void launch_adj_light(const Dev& d, bool f32, hipStream_t s);
 void launch_adj_heavy(const Dev& d, bool f32, int heavy_grid, hipStream_t s);
 void launch_pass_prep(const Dev& d, int pass, hipStream_t s);
-void launch_pass_carry(const Dev& d, int pass, hipStream_t s);
+void launch_pass_carry(const Dev& d, int pass, int check, hipStream_t s);
 // after `passes` expansion passes: pending work abandons the rest of the run
-void launch_expand_check(const Dev& d, int passes, hipStream_t s);
 void launch_expand(const Dev& d, int pass, int grid, int seeds, int threads, int budget,
                    hipStream_t s);
 void launch_merge(const Dev& d, int grid, hipStream_t s);
