@@ -457,11 +457,11 @@ hipError_t capture_run(dclust_handle* h, const Dev& d, const RunShape& shape, bo
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
   h->cap_phases = phase_times_requested();
-  hipError_t body = mark(h, E_START, true);
-  if (body == hipSuccess) {
-    enqueue_head(d, nchunks, s);
-    body = enqueue_index(h, d, d.n, nchunks, grid_cells, sort_passes, true);
-  }
+  // (the start event is recorded on the stream before each launch, outside
+  // the graph, where the device processes it while the host is still
+  // submitting the graph)
+  enqueue_head(d, nchunks, s);
+  hipError_t body = enqueue_index(h, d, d.n, nchunks, grid_cells, sort_passes, true);
   if (body == hipSuccess) body = enqueue_core(h, d, shape, f32, true);
   // work still pending after the last pass: the rest of the graph returns at
   // once and the host runs the call eagerly
@@ -833,6 +833,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
     };
     if (h->gexec && !fixed_mem) {
       HIPCHK(upload());
+      HIPCHK(hipEventRecord(h->ev[E_START], s));
       HIPCHK(hipGraphLaunch(h->gexec, s));
       HIPCHK(read_counters(h, d, s, ctl_h, stat_h, h->greadback));
       if (!ctl_h[C_ABANDON]) {
