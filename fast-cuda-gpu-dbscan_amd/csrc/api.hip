@@ -314,14 +314,16 @@ void drop_graph(dclust_handle* h) {
 }
 
 // Phase times of a replay: each event recorded inside a graph is a node that
-// leaves the device idle for about 5.5 us, so a captured run holds only the
-// events of the total and of the neighbour-count kernel, unless the
-// environment variable DCLUST_PHASE_TIMES=1 asks for every phase.
+// leaves the device idle for about 5.5 us.  The run's start and end events
+// are recorded on the stream around the graph launch, outside the graph; a
+// captured run holds only the two events of the neighbour-count kernel,
+// unless the environment variable DCLUST_PHASE_TIMES=1 asks for every phase.
 bool phase_times_requested() {
   const char* e = std::getenv("DCLUST_PHASE_TIMES");
   return e && e[0] == '1';
 }
-bool always_marked(int e) { return e == E_START || e == E_CORE0 || e == E_CORE || e == E_END; }
+bool outside_graph(int e) { return e == E_START || e == E_END; }
+bool always_in_graph(int e) { return e == E_CORE0 || e == E_CORE; }
 
 // Everything a launch sequence depends on besides the coordinates: the
 // device state (sizes, buffer and input/output pointers, switches), the
@@ -346,7 +348,7 @@ std::vector<uint8_t> launch_signature(const Dev& d, const RunShape& r, bool f32,
 
 // Phase event; inside a stream capture it becomes a node of the graph.
 hipError_t mark(dclust_handle* h, int e, bool graph, hipStream_t s = nullptr) {
-  if (graph && !h->cap_phases && !always_marked(e)) return hipSuccess;
+  if (graph && (outside_graph(e) || (!h->cap_phases && !always_in_graph(e)))) return hipSuccess;
   if (!s) s = h->stream;
   return graph ? hipEventRecordWithFlags(h->ev[e], s, hipEventRecordExternal)
                : hipEventRecord(h->ev[e], s);
@@ -457,9 +459,9 @@ hipError_t capture_run(dclust_handle* h, const Dev& d, const RunShape& shape, bo
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
   h->cap_phases = phase_times_requested();
-  // (the start event is recorded on the stream before each launch, outside
-  // the graph, where the device processes it while the host is still
-  // submitting the graph)
+  // (the start and end events are recorded on the stream around each
+  // launch, outside the graph: the start event is processed while the host
+  // is still submitting the graph)
   enqueue_head(d, nchunks, s);
   hipError_t body = enqueue_index(h, d, d.n, nchunks, grid_cells, sort_passes, true);
   if (body == hipSuccess) body = enqueue_core(h, d, shape, f32, true);
@@ -835,6 +837,7 @@ int run_device(dclust_handle* h, int32_t ninst, const double* d_xyz, const int64
       HIPCHK(upload());
       HIPCHK(hipEventRecord(h->ev[E_START], s));
       HIPCHK(hipGraphLaunch(h->gexec, s));
+      HIPCHK(hipEventRecord(h->ev[E_END], s));  // (after the counters' readback)
       HIPCHK(read_counters(h, d, s, ctl_h, stat_h, h->greadback));
       if (!ctl_h[C_ABANDON]) {
         if (ctl_h[C_ERR]) {
