@@ -774,11 +774,19 @@ def main():
                          "core_mismatches": int(np.sum(core != rcore)),
                          "oracle_asym_pairs": None if asym is None else int(asym),
                          "checked_on": f"the timed input ({how})"}
-    # the same clustering from host arrays (H2D + D2H included), one run
+    # the same clustering from host arrays (H2D + D2H included): two untimed
+    # calls (the first call of a new launch state runs eagerly, the second
+    # captures its graph), then the mean of three
+    def host_call():
+        h.run(x, info["eps"], info["minpts"], r=info["partitions_r"],
+              seeds=info["seed_list_size"], precision=prec)
+
+    for _ in range(2):
+        host_call()
     t0 = time.perf_counter()
-    h.run(x, info["eps"], info["minpts"], r=info["partitions_r"], seeds=info["seed_list_size"],
-          precision=prec)
-    out["pcie_inclusive_mpoints_s"] = round(n / (time.perf_counter() - t0) / 1e6, 2)
+    for _ in range(3):
+        host_call()
+    out["pcie_inclusive_mpoints_s"] = round(3 * n / (time.perf_counter() - t0) / 1e6, 2)
     if not args.no_batched:
         log("batched leg")
         out["batched"] = batched_leg(hip, h, max(2, args.steps // 2), 1,
