@@ -262,6 +262,47 @@ def test_graph_replay(handle, devmem, prec, monkeypatch):
     assert replayed[1:] == [0, 1, 1, 1, 0, 0, 1, 1]
 
 
+def test_graph_replay_phase_times(handle, devmem, monkeypatch):
+    # a replay records only the run's and the neighbour-count kernel's events:
+    # its other phase times read 0; DCLUST_PHASE_TIMES=1 (part of the launch
+    # state, so the graph is captured again) records every phase; the labels
+    # are the same either way
+    monkeypatch.delenv("DCLUST_GRAPH", raising=False)
+    monkeypatch.delenv("DCLUST_PHASE_TIMES", raising=False)
+    n, eps, mp = 30000, 0.02, 5
+    x = _blob_sample(5, n)
+    ref, _, _ = oracle.fast_dbscan(x, eps, mp)
+    p = dclust.make_params(3, eps, mp, precision=dclust.FP32_GUARD)
+    d_x = devmem.alloc(n * 3 * 8)
+    d_lab = devmem.alloc(n * 4)
+    devmem.h2d(d_x, x)
+    phases = ("t_rows_ms", "t_adj_ms", "t_expand_ms", "t_merge_ms", "t_far_ms", "t_border_ms",
+              "t_relabel_ms")
+    for env in (None, "1", None):
+        if env is None:
+            monkeypatch.delenv("DCLUST_PHASE_TIMES", raising=False)
+        else:
+            monkeypatch.setenv("DCLUST_PHASE_TIMES", env)
+        # a call of another size first drops any graph left by earlier calls
+        _check(handle, _blob_sample(9, n // 2), eps, mp, dclust.FP32_GUARD)
+        sts = []
+        for _ in range(3):
+            st = dict(handle.run_device(d_x, n, p, d_lab))
+            lab = devmem.download(np.empty(n, dtype=np.int32), d_lab)
+            assert np.array_equal(lab, ref)
+            sts.append(st)
+        assert [s["diag"][14] for s in sts] == [0, 1, 1]  # eager, then captured and replayed
+        last = sts[-1]
+        assert last["t_core_ms"] > 0 and last["t_total_ms"] >= last["t_index_ms"] + last["t_core_ms"]
+        if env is None:
+            assert all(last[k] == 0 for k in phases) and last["diag"][4] == 0
+        else:
+            assert all(last[k] > 0 for k in phases)
+            assert last["t_total_ms"] >= sum(last[k] for k in phases) + last["t_core_ms"]
+        # the eager runs record every phase event
+        assert all(sts[0][k] > 0 for k in phases)
+
+
 def test_graph_replay_switch(handle, monkeypatch):
     monkeypatch.setenv("DCLUST_GRAPH", "0")
     x = _blob_sample(4, 12000, dim=2)
