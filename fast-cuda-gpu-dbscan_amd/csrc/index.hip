@@ -449,53 +449,21 @@ __global__ __launch_bounds__(RS_T) void k_rs_hist(Dev d, int pass, int nblk) {
   }
   __syncthreads();
   d.hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+  // the pass runs unless the row scan finds one digit holding every key
+  // (no other block of this launch reads the flag of this pass)
+  if (blockIdx.x == 0 && threadIdx.x == 0) d.pass_scatter[pass] = 1;
 }
 
-// Digit totals, one block per digit (a sum over the digit's row of block
-// histograms; atomics from every block into 256 totals were slower).
-__global__ __launch_bounds__(256) void k_rs_total(Dev d, int pass, int nblk) {
-  if (!pass_active(d, pass)) return;
-  const uint32_t* row = d.hist + (int64_t)blockIdx.x * nblk;
-  uint32_t s = 0;
-  // eight loads in flight per thread (clamped indices, the excess not added)
-  for (int b0 = threadIdx.x; b0 < nblk; b0 += 8 * 256) {
-    uint32_t v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = row[b0 + u * 256 < nblk ? b0 + u * 256 : nblk - 1];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) s += b0 + u * 256 < nblk ? v[u] : 0u;
-  }
-  s = wave_sum(s);
-  __shared__ uint32_t red[4];
-  if (lane_id() == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) d.dtot[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
-}
-
-// Per-digit row scans, one block per digit: offs[d][b] = global start of
-// digit d in block b.  Every block also derives its digit's base (the keys
-// of all smaller digits) and the pass decision from the digit totals: a
-// pass in which every key has the same digit
-// leaves the data in order, so its scatter is skipped and the buffers are
-// not swapped (block 0 records the decision for the scatter and for the
-// buffer parity of later passes).
+// Per-digit row scans, one block per digit: offs[d][b] = keys of digit d in
+// the blocks before b; the block also publishes its digit's total (the
+// scatter derives the digit bases from the 256 totals).  A pass in which one
+// digit holds every key leaves the data in order: that digit's block unmarks
+// the pass, so its scatter is skipped and the buffers are not swapped.
 __global__ __launch_bounds__(256) void k_rs_rowscan(Dev d, int pass, int nblk) {
-  const uint32_t* tot = d.dtot;
+  if (!pass_active(d, pass)) return;
   __shared__ uint32_t s_w[4];
   __shared__ uint32_t s_carry;
-  __shared__ int s_skip;
-  const bool active = pass_active(d, pass);
-  const uint32_t v = tot[threadIdx.x];
-  if (threadIdx.x == 0) s_skip = 0;
-  __syncthreads();
-  if (active && (int64_t)v == d.n) s_skip = 1;  // every key has this digit
-  const uint32_t below = wave_sum(threadIdx.x < blockIdx.x ? v : 0u);
-  if (lane_id() == 0) s_w[threadIdx.x >> 6] = below;
-  __syncthreads();
-  if (threadIdx.x == 0) s_carry = s_w[0] + s_w[1] + s_w[2] + s_w[3];
-  if (blockIdx.x == 0 && threadIdx.x == 0) d.pass_scatter[pass] = (active && !s_skip) ? 1 : 0;
-  __syncthreads();
-  if (!active || s_skip) return;
+  if (threadIdx.x == 0) s_carry = 0;
   const uint32_t* row = d.hist + (int64_t)blockIdx.x * nblk;
   uint32_t* out = d.offs + (int64_t)blockIdx.x * nblk;
   // eight chunks' loads in flight per thread, then the chunks scanned in turn
@@ -525,6 +493,10 @@ __global__ __launch_bounds__(256) void k_rs_rowscan(Dev d, int pass, int nblk) {
     if (threadIdx.x == 0) s_carry += s_w[0] + s_w[1] + s_w[2] + s_w[3];
     __syncthreads();
   }
+  if (threadIdx.x == 0) {
+    d.dtot[blockIdx.x] = s_carry;
+    if ((int64_t)s_carry == d.n) d.pass_scatter[pass] = 0;  // every key has this digit
+  }
 }
 
 // Stable scatter.  Each wave owns a contiguous run of 64 * NI keys of the
@@ -549,7 +521,7 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) 
   __shared__ int32_t s_v[TILE];
   __shared__ uint32_t s_lstart[256], s_goff[256];
   __shared__ uint32_t s_wc[4][256];
-  __shared__ uint32_t s_ws[4];
+  __shared__ uint32_t s_ws[4], s_wt[4];
   const int tid = threadIdx.x, wv = tid >> 6;
   const int64_t t0 = (int64_t)blockIdx.x * TILE;
   const int64_t w0 = t0 + (int64_t)wv * (64 * NI) + lane_id();
@@ -566,16 +538,26 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(Dev d, int pass, int nblk) 
     vr[r] = vin[ic];
   }
   const uint32_t c = d.hist[(int64_t)tid * nblk + blockIdx.x];
-  s_goff[tid] = d.offs[(int64_t)tid * nblk + blockIdx.x];
+  const uint32_t off = d.offs[(int64_t)tid * nblk + blockIdx.x];
+  const uint32_t tot = d.dtot[tid];
 #pragma unroll
   for (int w = 0; w < 4; ++w) s_wc[w][tid] = 0;
-  // the tile's local digit starts: an exclusive scan of its digit counts
+  // the tile's local digit starts (an exclusive scan of its digit counts)
+  // and the digit bases (an exclusive scan of the pass's digit totals)
   const uint32_t inc = wave_incl_scan(c);
-  if (lane_id() == 63) s_ws[wv] = inc;
+  const uint32_t inct = wave_incl_scan(tot);
+  if (lane_id() == 63) {
+    s_ws[wv] = inc;
+    s_wt[wv] = inct;
+  }
   __syncthreads();
-  uint32_t lst = inc - c;
-  for (int w = 0; w < wv; ++w) lst += s_ws[w];
+  uint32_t lst = inc - c, base = inct - tot;
+  for (int w = 0; w < wv; ++w) {
+    lst += s_ws[w];
+    base += s_wt[w];
+  }
   s_lstart[tid] = lst;
+  s_goff[tid] = base + off;
   uint32_t lr[NI];
 #pragma unroll
   for (int r = 0; r < NI; ++r) {
@@ -1285,7 +1267,6 @@ static void index_sort(const Dev& d, int64_t n, int nchunks, int passes, hipStre
   const int nblk = (int)((n + RS_T * NI - 1) / (RS_T * NI));
   for (int p = 0; p < passes; ++p) {
     hipLaunchKernelGGL((k_rs_hist<K32, NI>), dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
-    hipLaunchKernelGGL(k_rs_total, dim3(256), dim3(256), 0, s, d, p, nblk);
     hipLaunchKernelGGL(k_rs_rowscan, dim3(256), dim3(256), 0, s, d, p, nblk);
     hipLaunchKernelGGL((k_rs_scatter<K32, NI>), dim3(nblk), dim3(RS_T), 0, s, d, p, nblk);
   }
