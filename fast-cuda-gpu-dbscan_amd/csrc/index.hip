@@ -13,7 +13,8 @@
 //                 ranges (INDEXING_ADJUSTMENT's per-leaf equal_range,
 //                 indexing.cu:23-51)
 //   k_gather      cell-sorted SoA coordinates (FP64 + fixed-point)
-//   k_rows        the 3^(D-1) neighbour rows of each cell as point ranges
+//   k_rows        the 5^(D-1) stencil lines of each cell (cells of side
+//                 eps/2, a 5^D stencil) as runs of point ranges
 //                 (searchPoints' level-by-level descent, dbscan.cu:390-474)
 #include "internal.h"
 #include "kernels.h"
