@@ -85,3 +85,23 @@ def test_device_entry_points_reject_bad_arguments_before_any_gpu_call():
         assert fn(None, None, 0, ctypes.byref(p), None, None) == dclust.E_PARAM
         assert fn(P(1), None, 0, None, None, None) == dclust.E_PARAM
         assert fn(P(1), None, -1, ctypes.byref(p), None, None) == dclust.E_PARAM
+
+
+def test_stats_view_is_a_mapping_over_the_struct():
+    # run_device returns a lazy mapping over dclust_stats: every field, the
+    # arrays as lists, a KeyError for an unknown name, and dict() of it equal
+    # to the struct's full conversion
+    st = dclust.Stats()
+    st.t_total_ms = 1.5
+    st.n_clusters = 7
+    st.diag[14] = 1
+    st.work[2] = 9
+    v = dclust.StatsView(st)
+    assert v["t_total_ms"] == 1.5 and v["n_clusters"] == 7
+    assert v["diag"][14] == 1 and isinstance(v["diag"], list) and len(v["diag"]) == 16
+    assert v["work"] == [0, 0, 9, 0]
+    assert dict(v) == st.as_dict()
+    assert set(v) == {k for k, _ in dclust.Stats._fields_}
+    assert "t_core_ms" in v and "nope" not in v
+    with pytest.raises(KeyError):
+        v["nope"]
